@@ -1,0 +1,162 @@
+/*
+ * liblife_hip.so -- MI355X (gfx950) implementation of the LIFE fluid and
+ * immersed-boundary coupling hot path, behind a plain C ABI.
+ *
+ * Two layers are exported:
+ *
+ *  1. life_grid_* / life_lbm_* / life_ibm_*: device lattice and marker
+ *     state with one entry point per reference method on the per-step path.
+ *     A maintainer of the reference keeps GridClass/ObjectsClass/FEM on the
+ *     host and replaces the bodies of those methods with these calls.
+ *
+ *  2. life_case_*: the library's own C++ host driver, which owns a lattice,
+ *     its bodies and their FP64 FEM, and runs the reference's main loop body
+ *     (GridClass::solver, Grid.cpp:25-33) on top of layer 1.
+ *
+ * Every function returns 0 on success and a non-zero code on failure;
+ * life_last_error() then returns the message (thread-local).  No C++
+ * exception crosses this boundary.  A handle is bound to one HIP device and
+ * one HIP stream and is not thread-safe; distinct handles are independent.
+ *
+ * Units and layouts follow the reference: lattice id = i*Ny + j (j fastest,
+ * Grid.cpp:70), populations in D2Q9 order v0..v8 (Grid.cpp:1249), host-side
+ * population arrays in the reference's AoS order f[id*9 + v], vectors as
+ * [id*2 + d], marker positions/velocities in physical units (IBMNode.h:52-53).
+ */
+#ifndef LIFE_HIP_H
+#define LIFE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* eLatType, inc/defs.h:52 */
+enum { LIFE_FLUID = 0, LIFE_WALL = 1, LIFE_VELOCITY = 2, LIFE_FREESLIP = 3, LIFE_PRESSURE = 4, LIFE_CONVECTIVE = 5 };
+/* eProfileType, inc/defs.h:53; LIFE_UNIFORM = PROFILE left undefined */
+enum { LIFE_UNIFORM = -1, LIFE_PARABOLIC = 0, LIFE_SHEAR = 1, LIFE_BOUNDARY_LAYER = 2 };
+/* Population storage. */
+enum {
+    LIFE_FP64 = 0,          /* plain method: FP64 populations, reference operation order */
+    LIFE_FP32_SHIFTED = 1   /* f_i - w_i*rho0 stored in FP32, FP32 collision, FP64 reductions */
+};
+
+/* Runtime form of inc/params.h:26-110 (same names and meanings). */
+typedef struct life_params {
+    int Nx, Ny;
+    double height_p, rho_p, nu_p;
+    double ux0_p, uy0_p;
+    double gravityX, gravityY, dpdx, dpdy;
+    int wall_left, wall_right, wall_bottom, wall_top;
+    int profile;
+    double uxInlet_p, uyInlet_p;
+    double alpha, delta;
+    double relaxMax, subTol;
+    double omega;            /* > 0: used directly; else derived from tStep (params.h:89-90) */
+    double tStep;
+    int central_moments;     /* CENTRAL_MOMENTS (params.h:32) */
+    double inlet_ramp;       /* INLET_RAMP seconds; <= 0: undefined (params.h:33) */
+    int uni_epsilon;         /* UNI_EPSILON (params.h:35) */
+    double initial_deflect;  /* INITIAL_DEFLECT; 0: undefined (params.h:37) */
+    double ref_nu, ref_rho, ref_P, ref_L, ref_U;
+} life_params;
+
+typedef struct life_grid life_grid;
+typedef struct life_case life_case;
+
+const char *life_last_error(void);
+/* Library version string. */
+const char *life_version(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int life_device_count(int *count);
+
+/* ---------------------------------------------------------------- layer 1 */
+
+/* Replaces GridClass::GridClass + initialiseGrid (Grid.cpp:1232-1289,
+ * 916-1062): allocate the device lattice for a case and set f = f_eq. */
+int life_grid_create(const life_params *p, int precision, int device, life_grid **out);
+int life_grid_destroy(life_grid *g);
+
+/* Derived scalars omega, tau, nu, Dx, Dt, Dm, Drho, c_s, c_s^2, c_s^4
+ * (Grid.cpp:1246-1260). */
+int life_grid_scalars(const life_grid *g, double out[10]);
+
+/* Replaces GridClass::lbmKernel (Grid.cpp:36-100) for time step t:
+ * convective speed, fused stream-collide, boundary conditions.  Enqueued on
+ * the grid's stream; returns without waiting. */
+int life_lbm_kernel(life_grid *g, int t);
+
+/* Materialise the post-step state in the reference layout (the arrays that
+ * GridClass::writeVTK/writeRestart read, Grid.cpp:790-913, 1163-1229).
+ * Any pointer may be NULL.  Blocks until the data is on the host. */
+int life_grid_download(life_grid *g, double *f_aos, double *rho, double *u, double *force_ibm);
+/* Load a full state (GridClass::readRestart, Grid.cpp:1072-1160).  The next
+ * step uses rho/u as given, as the reference does after a restart. */
+int life_grid_upload(life_grid *g, const double *f_aos, const double *rho, const double *u,
+                     const double *force_ibm);
+/* Wait for all work queued on the grid's stream. */
+int life_grid_sync(life_grid *g);
+
+/* Markers: replaces the IBMNodeClass arrays (IBMNode.h:45-70).  n markers in
+ * the reference's global order; body[k] is the index of marker k's body
+ * (bodies hold contiguous marker ranges); body_flex[b] is 0 for eFlexible,
+ * 1 for eRigid (defs.h:48). */
+int life_ibm_set_markers(life_grid *g, int n, const double *pos, const double *vel, const int *body,
+                         int nbodies, const int *body_flex);
+/* Update marker positions/velocities (after FEMBodyClass::updateIBMValues,
+ * FEMBody.cpp:147-195).  pos/vel may be NULL. */
+int life_ibm_update_markers(life_grid *g, const double *pos, const double *vel);
+/* IBMNodeClass::findSupport + computeDs (IBMNode.cpp:139-204) for all
+ * markers (flexible_only = 0, Objects.cpp:949-955) or only the markers of
+ * flexible bodies (flexible_only = 1, Objects.cpp:217-227). */
+int life_ibm_find_support(life_grid *g, int flexible_only);
+/* Epsilon matrix of ObjectsClass::computeEpsilon (Objects.cpp:258-302) for
+ * the system made of markers [first, first+n): A row-major n x n.  */
+int life_ibm_epsilon_matrix(life_grid *g, int first, int n, double *A);
+/* Set epsilon for all markers (Objects.cpp:309-310). */
+int life_ibm_set_epsilon(life_grid *g, const double *eps);
+/* ObjectsClass::ibmKernelInterp (Objects.cpp:102-117): interpolate rho and
+ * rho*u to every marker and compute its direct-forcing force; force (n*2,
+ * may be NULL) receives the marker forces.  Blocks when force != NULL. */
+int life_ibm_interpolate(life_grid *g, double *force);
+/* ObjectsClass::ibmKernelSpread (Objects.cpp:120-149): spread marker forces
+ * onto the lattice in marker order and refresh the support-site moments. */
+int life_ibm_spread(life_grid *g);
+/* Marker data for output and tests.  Any pointer may be NULL. */
+int life_ibm_download(life_grid *g, double *pos, double *vel, double *force, double *eps, double *ds,
+                      int *supp_count, int *supp_ij, double *supp_delta);
+
+/* Replaces Utils::solveLAPACK (Utils.cpp:288-311): solve A x = b for the
+ * trailing (n-bc) block of the row-major n x n matrix A, x[0..bc) = 0, with
+ * the library's canonical partial-pivot elimination (host, FP64; no GPU). */
+int life_solve_lapack(int n, int bc, const double *A, const double *b, double *x);
+
+/* ---------------------------------------------------------------- layer 2 */
+
+/* Build a case from params and geometry.config text (Objects.cpp:324-488),
+ * including body construction, overlap removal and initialiseObjects
+ * (Objects.cpp:867-961). */
+int life_case_create(const life_params *p, const char *geometry, int precision, int device, life_case **out);
+int life_case_destroy(life_case *c);
+/* nsteps iterations of the main loop body (main.cpp:70-73):
+ * t++, lbmKernel, objectKernel. */
+int life_case_step(life_case *c, int nsteps);
+/* As life_case_step, also returning the elapsed device time of the steps
+ * (ms, HIP events on the case stream) and of the fused stream-collide kernel
+ * launches alone (ms, summed). */
+int life_case_step_timed(life_case *c, int nsteps, double *ms_total, double *ms_kernel);
+int life_case_time(const life_case *c, int *t);
+life_grid *life_case_grid(life_case *c);
+int life_case_num_markers(const life_case *c, int *n);
+int life_case_num_bodies(const life_case *c, int *n);
+/* body info: ID, flex, nmarkers, first marker, FEM dofs */
+int life_case_body_info(const life_case *c, int b, int info[5]);
+int life_case_get_fem(const life_case *c, int b, double *U, double *Udot, double *Udotdot);
+/* subIt, subRes, relax of the last objectKernel (Objects.cpp:26-60) */
+int life_case_coupling_info(const life_case *c, double out[3]);
+/* ObjectsClass::writeTotalForces quantities (Objects.cpp:517-534): Fx, Fy, Cx, Cy */
+int life_case_total_force(life_case *c, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""Runtime case description: the reference's compile-time params.h as data.
+
+LIFE bakes each case into ``inc/params.h`` (params.h:26-110) and reads bodies
+from ``input/geometry.config`` (Objects.cpp:324-488).  Serving mixes grid
+sizes, so here both are runtime values: :class:`CaseParams` carries the
+params.h constants with the same names and meanings, and the geometry stays
+the reference's text format (one ``CIRCLE``/``FILAMENT`` line per body group).
+
+The named workloads below are the benchmark and parity cases.  They are all
+set up in lattice units (``Dx = 1``, ``Dt = 1``): ``height_p = Ny - 1``,
+``rho_p = 1`` and ``nu_p`` equal to the lattice viscosity that the solver
+derives from ``omega`` (Grid.cpp:1255-1258), so positions, lengths and
+velocities below are in lattice units.
+"""
+import dataclasses
+import math
+import random
+
+# eLatType (inc/defs.h:52)
+FLUID, WALL, VELOCITY, FREESLIP, PRESSURE, CONVECTIVE = range(6)
+# eProfileType (inc/defs.h:53); UNIFORM = PROFILE undefined
+UNIFORM, PARABOLIC, SHEAR, BOUNDARY_LAYER = -1, 0, 1, 2
+
+
+@dataclasses.dataclass
+class CaseParams:
+    Nx: int
+    Ny: int
+    height_p: float
+    rho_p: float
+    nu_p: float
+    ux0_p: float = 0.0
+    uy0_p: float = 0.0
+    gravityX: float = 0.0
+    gravityY: float = 0.0
+    dpdx: float = 0.0
+    dpdy: float = 0.0
+    wall_left: int = VELOCITY
+    wall_right: int = PRESSURE
+    wall_bottom: int = WALL
+    wall_top: int = WALL
+    profile: int = PARABOLIC
+    uxInlet_p: float = 1.0
+    uyInlet_p: float = 0.0
+    alpha: float = 0.25
+    delta: float = 0.5
+    relaxMax: float = 1.0
+    subTol: float = 1e-8
+    omega: float = 0.0          # > 0: set directly (params.h:85)
+    tStep: float = 0.0          # used when omega <= 0 (params.h:89-90)
+    central_moments: int = 0
+    inlet_ramp: float = 0.0     # INLET_RAMP, <= 0: undefined
+    uni_epsilon: int = 0
+    initial_deflect: float = 0.0
+    ref_nu: float = 0.0
+    ref_rho: float = 1.0
+    ref_P: float = 0.0
+    ref_L: float = 1.0
+    ref_U: float = 1.0
+
+    def replace(self, **kw):
+        return dataclasses.replace(self, **kw)
+
+
+def lattice_viscosity(omega):
+    """nu = (1/omega - 1/2) c_s^2 with the solver's own rounding (Grid.cpp:1247, 1255-1256)."""
+    cs = 1.0 / math.sqrt(3.0)
+    tau = 1.0 / omega
+    return (tau - 0.5) * (cs * cs)
+
+
+def lattice_case(Nx, Ny, tau, **kw):
+    """CaseParams in lattice units: Dx = 1 and Dt = 1 exactly."""
+    omega = 1.0 / tau
+    nu = lattice_viscosity(omega)
+    base = dict(Nx=Nx, Ny=Ny, height_p=float(Ny - 1), rho_p=1.0, nu_p=nu, omega=omega, ref_nu=nu)
+    base.update(kw)
+    return CaseParams(**base)
+
+
+# ----------------------------------------------------------------------------
+# Named workloads
+# ----------------------------------------------------------------------------
+
+def lid_driven_cavity(N=256, tau=0.6, u_lid=0.05, central_moments=0):
+    """Lid-driven cavity, N x N, BGK, regularised no-slip walls, uniform lid
+    velocity on the top wall, f = f_eq(rho=1, u=0).  Re = u_lid (N-1) / nu."""
+    p = lattice_case(N, N, tau, wall_left=WALL, wall_right=WALL, wall_bottom=WALL, wall_top=VELOCITY,
+                     profile=UNIFORM, uxInlet_p=u_lid, uyInlet_p=0.0, central_moments=central_moments,
+                     ref_L=float(N - 1), ref_U=u_lid)
+    return p, ""
+
+
+def cylinder_channel(Nx=1024, Ny=256, D=40.0, centre=(200.0, 128.0), u_max=0.04, Re=100.0,
+                     ramp_steps=1000.0):
+    """Rigid cylinder in a channel: central moments, parabolic inlet (peak
+    u_max, so mean = u_max / 1.5), convective outlet, no-slip walls, one
+    rigid CIRCLE, per-body epsilon.  Re = mean * D / nu."""
+    u_mean = u_max / 1.5
+    nu = u_mean * D / Re
+    tau = 3.0 * nu + 0.5
+    p = lattice_case(Nx, Ny, tau, wall_left=VELOCITY, wall_right=CONVECTIVE, wall_bottom=WALL, wall_top=WALL,
+                     profile=PARABOLIC, uxInlet_p=u_mean, central_moments=1, inlet_ramp=ramp_steps,
+                     ref_L=D, ref_U=u_mean)
+    geom = "CIRCLE 1 0 %.17g %.17g 0.0 0.0 %.17g\n" % (centre[0], centre[1], D / 2.0)
+    return p, geom
+
+
+def flexible_fsi(Nx=2048, Ny=410, D=80.0, centre=(200.0, 200.0), L=280.0, h=16.0, n_el=40,
+                 density_ratio=10.0, stiffness=1.4e3, u_mean=0.04, Re=100.0, ramp_steps=2000.0,
+                 uni_epsilon=1, central_moments=1):
+    """Turek-Hron-type FSI: rigid cylinder with an attached clamped flexible
+    plate (corotational beam), Newmark alpha = 0.25, block Gauss-Seidel with
+    Aitken relaxation, subTol = 1e-8.  E = stiffness * rho_f * u_mean^2."""
+    nu = u_mean * D / Re
+    tau = 3.0 * nu + 0.5
+    p = lattice_case(Nx, Ny, tau, wall_left=VELOCITY, wall_right=PRESSURE, wall_bottom=WALL, wall_top=WALL,
+                     profile=PARABOLIC, uxInlet_p=u_mean, central_moments=central_moments,
+                     inlet_ramp=ramp_steps, uni_epsilon=uni_epsilon, ref_L=D, ref_U=u_mean)
+    E = stiffness * 1.0 * u_mean * u_mean
+    x0 = centre[0] + D / 2.0
+    geom = ("CIRCLE 1 0 %.17g %.17g 0.0 0.0 %.17g\n" % (centre[0], centre[1], D / 2.0) +
+            "FILAMENT 1 1 %.17g %.17g 0.0 0.0 %.17g %.17g 0.0 FLEXIBLE %d CLAMPED %.17g %.17g\n"
+            % (x0, centre[1], L, h, n_el, density_ratio, E))
+    return p, geom
+
+
+def canopy(Nx=8192, Ny=4096, n_filaments=64, L=200.0, n_el=50, min_spacing=60.0, tau=0.6, u_in=0.05,
+           h=1.0, density_ratio=10.0, kb_range=(1e-3, 1e-1), seed=20261202, x_range=None,
+           ramp_steps=2000.0):
+    """Bandwidth-roofline canopy: BGK, uniform inlet, convective outlet,
+    no-slip bottom, free-slip top (as the reference's Honami case), and
+    n_filaments clamped vertical flexible filaments rooted on the bottom wall
+    at seeded random x positions with a minimum spacing.  Bending stiffness
+    K_B = E I / (rho_f U^2 L^3), log-uniform; I = Dx h^3 / 12."""
+    rng = random.Random(seed)
+    lo, hi = x_range if x_range else (0.1 * Nx, 0.6 * Nx)
+    xs = []
+    tries = 0
+    while len(xs) < n_filaments:
+        tries += 1
+        if tries > 100000:
+            raise ValueError("cannot place %d filaments with spacing %g" % (n_filaments, min_spacing))
+        x = float(round(rng.uniform(lo, hi)))
+        if all(abs(x - y) >= min_spacing for y in xs):
+            xs.append(x)
+    xs.sort()
+    I = 1.0 * h ** 3 / 12.0
+    lines = []
+    for k, x in enumerate(xs):
+        kb = math.exp(rng.uniform(math.log(kb_range[0]), math.log(kb_range[1])))
+        E = kb * 1.0 * u_in * u_in * L ** 3 / I
+        lines.append("FILAMENT 1 %d %.17g 0.0 0.0 0.0 %.17g %.17g 90.0 FLEXIBLE %d CLAMPED %.17g %.17g"
+                     % (k, x, L, h, n_el, density_ratio, E))
+    p = lattice_case(Nx, Ny, tau, wall_left=VELOCITY, wall_right=CONVECTIVE, wall_bottom=WALL,
+                     wall_top=FREESLIP, profile=UNIFORM, uxInlet_p=u_in, central_moments=0,
+                     inlet_ramp=ramp_steps, uni_epsilon=0, ref_L=L, ref_U=u_in)
+    return p, "\n".join(lines) + ("\n" if lines else "")
+
+
+def inverted_flag(Nx=512, Ny=256, length_frac=0.25, n_el=20, Re=200.0, kb=0.5, u_in=0.04,
+                  density_ratio=10.0, h=1.0, central_moments=0):
+    """Inverted flag (the reference's InvertedFlag case): one flexible
+    plate clamped at its trailing edge (angle 180 degrees), uniform inflow,
+    convective outlet, velocity top/bottom.  Plate length = length_frac * Ny."""
+    L = float(int(length_frac * Ny))
+    nu = u_in * L / Re
+    tau = 3.0 * nu + 0.5
+    I = 1.0 * h ** 3 / 12.0
+    E = kb * 1.0 * u_in * u_in * L ** 3 / I
+    x0 = 0.4 * Nx + L
+    y0 = 0.5 * (Ny - 1)
+    p = lattice_case(Nx, Ny, tau, wall_left=VELOCITY, wall_right=CONVECTIVE, wall_bottom=VELOCITY,
+                     wall_top=VELOCITY, profile=UNIFORM, uxInlet_p=u_in, ux0_p=u_in,
+                     central_moments=central_moments, subTol=1e-5, ref_L=L, ref_U=u_in)
+    geom = ("FILAMENT 1 0 %.17g %.17g 0.0 0.0 %.17g %.17g 180.0 FLEXIBLE %d CLAMPED %.17g %.17g\n"
+            % (x0, y0, L, h, n_el, density_ratio, E))
+    return p, geom

@@ -1,0 +1,458 @@
+// Layer 2: the library's own host driver for one LIFE case (life_case_*).
+//
+// Owns a life_grid, the bodies read from geometry.config text, their FP64
+// FEM and the strongly coupled subiteration loop, and runs the reference's
+// main-loop body (GridClass::solver, Grid.cpp:25-33) on top of the layer-1
+// entry points.  FEM, the epsilon solve and the Aitken relaxation stay FP64
+// on the host; everything on the lattice runs on the device.
+//
+// Compiled with -ffp-contract=off.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fem.h"
+#include "lu.h"
+#include "runtime.h"
+
+using namespace life;
+
+namespace {
+
+struct Body {
+    int ID;
+    int flex;  // 0 flexible, 1 rigid (eFlexibleType, defs.h:48)
+    int kind;  // 0 circle, 1 filament
+    int first = 0, n = 0;
+    std::unique_ptr<FemBody> fem;
+    // construction data kept until the markers are final
+    std::vector<double> mpos;
+    double fpos[2] = {0, 0}, fgeom[2] = {0, 0}, fangle = 0, frho = 0, fE = 0;
+    std::string fnel, fbc;
+};
+
+}  // namespace
+
+struct life_case {
+    life_params p{};
+    life_grid* g = nullptr;
+    CaseScalars cs{};
+    int t = 0;
+    std::vector<Body> bodies;
+    int nm = 0;
+    std::vector<double> pos, vel, force, eps, ds;
+    std::vector<int> mbody;
+    bool hasIBM = false, hasFlex = false;
+    int subIt = 0, simDOFs = 0;
+    double subRes = 0.0, subNum = 0.0, subDen = 0.0, relax = 1.0;
+    std::vector<double> A;
+    std::vector<double> luw;
+    std::vector<int> luiw;
+    std::vector<double> rhs, sol;
+    bool forces_on_host = false;
+
+    MarkerView view(const Body& b) {
+        return MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};
+    }
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return 2;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+// IBMBodyClass circle constructor, IBMBody.cpp:45-76.
+void build_circle(life_case* c, int ID, const double pos[2], double radius) {
+    Body b;
+    b.ID = ID;
+    b.flex = 1;
+    b.kind = 0;
+    const int n = (int)std::floor(2.0 * M_PI * radius / c->cs.Dx);
+    for (int i = 0; i < n; i++) {
+        b.mpos.push_back(pos[0] + radius * std::cos(i * 2.0 * M_PI / n));
+        b.mpos.push_back(pos[1] + radius * std::sin(i * 2.0 * M_PI / n));
+    }
+    c->bodies.push_back(std::move(b));
+}
+
+// IBMBodyClass filament constructor, IBMBody.cpp:79-135.
+void build_filament(life_case* c, int ID, const double pos[2], const double geom[2], double angleDeg,
+                    const std::string& flex, const std::string& nEl, const std::string& bc, double rho, double E) {
+    Body b;
+    b.ID = ID;
+    if (flex == "FLEXIBLE") b.flex = 0;
+    else if (flex == "RIGID") b.flex = 1;
+    else throw std::runtime_error("Not a proper flexible type set");
+    b.kind = 1;
+    const double L = geom[0];
+    const double ang = angleDeg * M_PI / 180.0;
+    const double ca = std::cos(ang), sa = std::sin(ang);
+    const int n = (int)std::floor(L / c->cs.Dx) + 1;
+    for (int i = 0; i < n; i++) {
+        const double px = i * L / (n - 1), py = 0.0;
+        const double rx = 0.0 + ca * px + (-sa) * py, ry = 0.0 + sa * px + ca * py;
+        b.mpos.push_back(pos[0] + rx);
+        b.mpos.push_back(pos[1] + ry);
+    }
+    b.fpos[0] = pos[0];
+    b.fpos[1] = pos[1];
+    b.fgeom[0] = geom[0];
+    b.fgeom[1] = geom[1];
+    b.fangle = ang;
+    b.fnel = nEl;
+    b.fbc = bc;
+    b.frho = rho;
+    b.fE = E;
+    c->bodies.push_back(std::move(b));
+}
+
+// ObjectsClass::geometryReadIn, Objects.cpp:324-488, on text.
+void parse_geometry(life_case* c, const char* text) {
+    if (!text) return;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        size_t k = line.find_first_not_of(" \t\r");
+        if (k == std::string::npos || line[k] == '#') continue;
+        std::istringstream ls(line.substr(k));
+        std::string kw;
+        int number, ID;
+        double st[2], sp[2];
+        if (!(ls >> kw >> number >> ID >> st[0] >> st[1] >> sp[0] >> sp[1]))
+            throw std::runtime_error("malformed geometry line: " + line);
+        if (kw == "CIRCLE") {
+            double r;
+            if (!(ls >> r)) throw std::runtime_error("malformed CIRCLE line: " + line);
+            for (int i = 0; i < number; i++) {
+                const double pos[2] = {st[0] + i * sp[0], st[1] + i * sp[1]};
+                build_circle(c, ID++, pos, r);
+            }
+        } else if (kw == "FILAMENT") {
+            double g[2], ang, rho, E;
+            std::string flex, nel, bc;
+            if (!(ls >> g[0] >> g[1] >> ang >> flex >> nel >> bc >> rho >> E))
+                throw std::runtime_error("malformed FILAMENT line: " + line);
+            for (int i = 0; i < number; i++) {
+                const double pos[2] = {st[0] + i * sp[0], st[1] + i * sp[1]};
+                build_filament(c, ID++, pos, g, ang, flex, nel, bc, rho, E);
+            }
+        } else {
+            throw std::runtime_error("unknown body keyword '" + kw + "'");
+        }
+    }
+    for (size_t i = 0; i < c->bodies.size(); i++)
+        for (size_t j = i + 1; j < c->bodies.size(); j++)
+            if (c->bodies[i].ID == c->bodies[j].ID) throw std::runtime_error("Duplicate body IDs in geometry.config");
+}
+
+// Lay the markers out in global order and drop overlapping ones
+// (ObjectsClass::removeOverlapMarkers, Objects.cpp:867-938).
+void finalise_markers(life_case* c) {
+    std::vector<double> P;
+    std::vector<int> B;
+    for (size_t b = 0; b < c->bodies.size(); b++)
+        for (size_t k = 0; k < c->bodies[b].mpos.size() / 2; k++) {
+            P.push_back(c->bodies[b].mpos[2 * k]);
+            P.push_back(c->bodies[b].mpos[2 * k + 1]);
+            B.push_back((int)b);
+        }
+    const int nm = (int)B.size();
+    std::vector<char> erase(nm, 0);
+    for (int a = 0; a < nm; a++)
+        for (int q = a + 1; q < nm; q++) {
+            const double d[2] = {P[2 * a] - P[2 * q], P[2 * a + 1] - P[2 * q + 1]};
+            const double mag = std::sqrt(0.0 + d[0] * d[0] + d[1] * d[1]) / c->cs.Dx;
+            if (mag < 0.5) {
+                const int fa = c->bodies[B[a]].flex, fb = c->bodies[B[q]].flex;
+                if (fa == 0 && fb == 0) throw std::runtime_error("Cannot place two flexible bodies so close to each other");
+                erase[(fa == 0 && fb == 1) ? q : a] = 1;
+            }
+        }
+    for (auto& b : c->bodies) b.n = 0;
+    for (int a = 0; a < nm; a++) {
+        if (erase[a]) continue;
+        Body& b = c->bodies[B[a]];
+        if (b.n == 0) b.first = (int)c->mbody.size();
+        b.n++;
+        c->pos.push_back(P[2 * a]);
+        c->pos.push_back(P[2 * a + 1]);
+        c->mbody.push_back(B[a]);
+    }
+    c->nm = (int)c->mbody.size();
+    c->vel.assign(2 * (size_t)c->nm, 0.0);
+    c->force.assign(2 * (size_t)c->nm, 0.0);
+    c->eps.assign(c->nm, 0.0);
+    c->ds.assign(c->nm, 0.0);
+    for (auto& b : c->bodies) b.mpos.clear();
+}
+
+// ObjectsClass::computeEpsilon (Objects.cpp:235-321): device matrix, host
+// canonical solve of A eps = 1.
+void compute_epsilon(life_case* c, bool initial) {
+    auto solve = [&](int first, int n) {
+        c->A.resize((size_t)n * n);
+        ibm_epsilon_matrix(c->g, first, n, c->A.data());
+        c->rhs.assign(n, 1.0);
+        c->sol.assign(n, 0.0);
+        lu_solve_sparse(n, 0, c->A.data(), c->rhs.data(), c->sol.data(), c->luw, c->luiw);
+        std::memcpy(&c->eps[first], c->sol.data(), sizeof(double) * n);
+    };
+    if (c->p.uni_epsilon) {
+        solve(0, c->nm);
+    } else {
+        for (auto& b : c->bodies)
+            if ((initial || b.flex == 0) && b.n > 0) solve(b.first, b.n);
+    }
+    LIFE_HIP(hipMemcpyAsync(c->g->md.eps, c->eps.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice, c->g->stream));
+}
+
+void download_forces(life_case* c) {
+    LIFE_HIP(hipMemcpyAsync(c->force.data(), c->g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
+                            c->g->stream));
+    LIFE_HIP(hipStreamSynchronize(c->g->stream));
+}
+
+// ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
+void recompute_object_vals(life_case* c) {
+    if (c->subIt == 0) {
+        for (auto& b : c->bodies)
+            if (b.flex == 0) {
+                b.fem->reset_values();
+                b.fem->predictor(c->t, c->view(b));
+            }
+    } else {
+        if (c->subIt == 1) {
+            const double a = std::fabs(c->relax), r = a < c->p.relaxMax ? a : c->p.relaxMax;
+            c->relax = (double)(((0.0 < c->relax) - (c->relax < 0.0)) * r);
+        } else {
+            c->relax = -c->relax * c->subNum / c->subDen;
+        }
+        for (auto& b : c->bodies)
+            if (b.flex == 0) b.fem->relax(c->relax, c->view(b));
+    }
+    life_grid* g = c->g;
+    LIFE_HIP(hipMemcpyAsync(g->md.pos, c->pos.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
+    LIFE_HIP(hipMemcpyAsync(g->md.vel, c->vel.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
+    ibm_find_support(g, 1);
+    compute_epsilon(c, false);
+}
+
+// ObjectsClass::objectKernel, Objects.cpp:26-60.
+void object_kernel(life_case* c) {
+    const int MAXIT = 20;
+    c->subIt = 0;
+    c->forces_on_host = false;
+    do {
+        if (c->hasFlex) recompute_object_vals(c);
+        ibm_interpolate(c->g);
+        if (!c->hasFlex) break;
+        download_forces(c);
+        // femKernel, Objects.cpp:63-99 (ORDERED: sums in body order)
+        double res = 0.0, num = 0.0, den = 0.0;
+        for (auto& b : c->bodies)
+            if (b.flex == 0) {
+                b.fem->dynamic(c->view(b));
+                res += b.fem->sub_res();
+                num += b.fem->sub_num();
+                den += b.fem->sub_den();
+            }
+        c->subRes = std::sqrt(res) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
+        c->subNum = num;
+        c->subDen = den;
+        c->subIt++;
+    } while (c->subIt < MAXIT && c->subRes > c->p.subTol);
+    ibm_spread(c->g);
+    if (c->hasFlex) {
+        // the last FEM solve moved the markers (FEMBody.cpp:63-64); the
+        // spread above used the supports found before it, as the reference
+        // does, and the device copy now takes the final positions for output
+        LIFE_HIP(hipMemcpyAsync(c->g->md.pos, c->pos.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice,
+                                c->g->stream));
+        LIFE_HIP(hipMemcpyAsync(c->g->md.vel, c->vel.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice,
+                                c->g->stream));
+        c->forces_on_host = true;
+    }
+    if (c->subIt == MAXIT) throw std::runtime_error("Subiteration scheme hit 20 iterations");
+}
+
+void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, double* kernel_ms) {
+    c->t++;
+    lbm_step(c->g, c->t, ev_begin, ev_end);
+    if (c->hasIBM) object_kernel(c);
+    if (kernel_ms && ev_begin) {
+        LIFE_HIP(hipEventSynchronize(ev_end));
+        float ms = 0.0f;
+        LIFE_HIP(hipEventElapsedTime(&ms, ev_begin, ev_end));
+        *kernel_ms += ms;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int life_case_create(const life_params* p, const char* geometry, int precision, int device, life_case** out) {
+    *out = nullptr;
+    life_case* c = new life_case();
+    int rc = guarded([&] {
+        c->p = *p;
+        c->cs = derive_scalars(*p);
+        c->relax = p->relaxMax;
+        if (p->initial_deflect != 0.0) throw std::runtime_error("INITIAL_DEFLECT is not supported");
+        parse_geometry(c, geometry);
+        finalise_markers(c);
+        c->hasIBM = !c->bodies.empty();
+        FemScalars fs{c->cs.Dx, c->cs.Dt, c->cs.Dm, p->alpha, p->delta, p->gravityX, p->gravityY, p->ref_L};
+        for (auto& b : c->bodies) {
+            if (b.flex == 0) {
+                c->hasFlex = true;
+                b.fem.reset(new FemBody(fs, b.fpos, b.fgeom, b.fangle, b.fnel, b.fbc, b.frho, b.fE, b.n));
+                c->simDOFs += b.fem->ndof();
+            }
+        }
+        if (life_grid_create(p, precision, device, &c->g)) throw std::runtime_error(life_last_error());
+        if (c->hasIBM) {
+            std::vector<int> flex(c->bodies.size());
+            for (size_t b = 0; b < c->bodies.size(); b++) flex[b] = c->bodies[b].flex;
+            if (life_ibm_set_markers(c->g, c->nm, c->pos.data(), c->vel.data(), c->mbody.data(),
+                                     (int)c->bodies.size(), flex.data()))
+                throw std::runtime_error(life_last_error());
+            // ObjectsClass::initialiseObjects, Objects.cpp:941-961
+            ibm_find_support(c->g, 0);
+            compute_epsilon(c, true);
+            LIFE_HIP(hipMemcpyAsync(c->ds.data(), c->g->md.ds, sizeof(double) * c->nm, hipMemcpyDeviceToHost,
+                                    c->g->stream));
+            LIFE_HIP(hipStreamSynchronize(c->g->stream));
+            check_flags(c->g);
+        }
+    });
+    if (rc) {
+        life_case_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int life_case_destroy(life_case* c) {
+    if (!c) return 0;
+    int rc = 0;
+    if (c->g) rc = life_grid_destroy(c->g);
+    delete c;
+    return rc;
+}
+
+int life_case_step(life_case* c, int nsteps) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(c->g->device));
+        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr, nullptr);
+        LIFE_HIP(hipStreamSynchronize(c->g->stream));
+        if (c->hasIBM) check_flags(c->g);
+    });
+}
+
+int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_kernel) {
+    return guarded([&] {
+        life_grid* g = c->g;
+        LIFE_HIP(hipSetDevice(g->device));
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        double kms = 0.0;
+        LIFE_HIP(hipEventRecord(g->ev_a, g->stream));
+        for (int s = 0; s < nsteps; s++) case_step(c, g->ev_k0, g->ev_k1, &kms);
+        LIFE_HIP(hipEventRecord(g->ev_b, g->stream));
+        LIFE_HIP(hipEventSynchronize(g->ev_b));
+        float ms = 0.0f;
+        LIFE_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
+        if (ms_total) *ms_total = ms;
+        if (ms_kernel) *ms_kernel = kms;
+        if (c->hasIBM) check_flags(g);
+    });
+}
+
+int life_case_time(const life_case* c, int* t) {
+    *t = c->t;
+    return 0;
+}
+
+life_grid* life_case_grid(life_case* c) { return c->g; }
+
+int life_case_num_markers(const life_case* c, int* n) {
+    *n = c->nm;
+    return 0;
+}
+
+int life_case_num_bodies(const life_case* c, int* n) {
+    *n = (int)c->bodies.size();
+    return 0;
+}
+
+int life_case_body_info(const life_case* c, int b, int info[5]) {
+    return guarded([&] {
+        if (b < 0 || b >= (int)c->bodies.size()) throw std::runtime_error("body index out of range");
+        const Body& B = c->bodies[b];
+        info[0] = B.ID;
+        info[1] = B.flex;
+        info[2] = B.n;
+        info[3] = B.n ? B.first : -1;
+        info[4] = B.fem ? B.fem->ndof() : 0;
+    });
+}
+
+int life_case_get_fem(const life_case* c, int b, double* U, double* Udot, double* Udotdot) {
+    return guarded([&] {
+        if (b < 0 || b >= (int)c->bodies.size() || !c->bodies[b].fem) throw std::runtime_error("not a flexible body");
+        const FemBody& F = *c->bodies[b].fem;
+        const size_t n = (size_t)F.ndof() * sizeof(double);
+        if (U) std::memcpy(U, F.U.data(), n);
+        if (Udot) std::memcpy(Udot, F.Ud.data(), n);
+        if (Udotdot) std::memcpy(Udotdot, F.Udd.data(), n);
+    });
+}
+
+int life_case_coupling_info(const life_case* c, double out[3]) {
+    out[0] = c->subIt;
+    out[1] = c->subRes;
+    out[2] = c->relax;
+    return 0;
+}
+
+int life_case_total_force(life_case* c, double out[4]) {
+    return guarded([&] {
+        life_grid* g = c->g;
+        LIFE_HIP(hipSetDevice(g->device));
+        if (c->nm) {
+            LIFE_HIP(hipMemcpyAsync(c->force.data(), g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
+                                    g->stream));
+            LIFE_HIP(hipMemcpyAsync(c->ds.data(), g->md.ds, sizeof(double) * c->nm, hipMemcpyDeviceToHost, g->stream));
+            LIFE_HIP(hipStreamSynchronize(g->stream));
+        }
+        // ObjectsClass::writeTotalForces, Objects.cpp:517-534
+        const CaseScalars& s = c->cs;
+        const double scale = s.Dm * s.Dx / (s.Dt * s.Dt) * 1.0 / s.Dx;
+        double F[2] = {0.0, 0.0};
+        for (int n = 0; n < c->nm; n++)
+            for (int d = 0; d < 2; d++) F[d] -= c->force[2 * n + d] * 1.0 * c->eps[n] * c->ds[n] * scale;
+        const double ND = 0.5 * c->p.ref_rho * (c->p.ref_U * c->p.ref_U) * c->p.ref_L;
+        out[0] = F[0];
+        out[1] = F[1];
+        out[2] = F[0] / ND;
+        out[3] = F[1] / ND;
+    });
+}
+
+}  // extern "C"

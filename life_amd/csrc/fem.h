@@ -1,0 +1,102 @@
+// FP64 corotational Euler-Bernoulli beam (host side of the coupling).
+//
+// Restates FEMBodyClass / FEMElementClass / FEMNodeClass (FEMBody.cpp,
+// FEMElement.cpp, FEMNode.cpp): 3 DOF per node, consistent mass, linear +
+// geometric stiffness in a corotating frame, Newmark-beta time integration
+// and Newton-Raphson iterations.  The global matrices are assembled in
+// banded storage (an element couples DOFs at most 5 apart) and solved with
+// the banded form of the canonical elimination, which performs the same
+// floating-point operations as the reference's dense solve on the same
+// matrix; assembly and products keep the reference's accumulation order.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "lu.h"
+
+namespace life {
+
+// Marker arrays of the owning body (global marker order, physical units).
+struct MarkerView {
+    double* pos;          // 2 per marker
+    double* vel;          // 2 per marker
+    const double* force;  // 2 per marker (lattice units)
+    const double* eps;    // 1 per marker
+    int first, n;
+};
+
+struct FemScalars {
+    double Dx, Dt, Dm;
+    double alpha, delta;  // Newmark (params.h:76-77)
+    double gravityX, gravityY;
+    double ref_L;
+};
+
+class FemBody {
+  public:
+    // FEMBodyClass constructor (FEMBody.cpp:487-560); angle in radians.
+    FemBody(const FemScalars& s, const double pos[2], const double geom[2], double angle, const std::string& nEl,
+            const std::string& bc, double rho, double E, int n_markers);
+
+    void reset_values();                          // FEMBody.cpp:341-349
+    void predictor(int t, MarkerView mv);         // FEMBody.cpp:259-289
+    void relax(double omega, MarkerView mv);      // Objects.cpp:198-210
+    void dynamic(MarkerView mv);                  // FEMBody.cpp:26-68
+
+    int ndof() const { return ndof_; }
+    int bc_dofs() const { return nbc_; }
+    double sub_res() const { return subRes_; }
+    double sub_num() const { return subNum_; }
+    double sub_den() const { return subDen_; }
+    int it_nr() const { return itNR_; }
+    double res_nr() const { return resNR_; }
+    double L0() const { return L0_; }
+    double angle0() const { return angle0_; }
+    double t_crit() const { return tCrit_; }
+
+    std::vector<double> U, Ud, Udd, Un, Unm1, Unm2, Udn, Uddn, Ukm1, Rk, Rkm1, R, F, delU;
+
+  private:
+    struct Node {
+        double pos0[2], pos[2], angle0, angle;
+    };
+    struct Map {
+        int node;
+        double z1, z2;
+    };
+    struct Elem {
+        double L0, L, angle, A, I, E, rho;
+        double T[6][6], M[6][6], KL[6][6], KNL[6][6], R[6], F[6];
+        int n0, n1;
+        std::vector<Map> map;
+    };
+    struct PosMap {
+        int el;
+        double zeta;
+    };
+
+    void update_fem();                 // FEMBody.cpp:198-223
+    void update_ibm(MarkerView mv);    // FEMBody.cpp:147-195
+    void finish_newmark();             // FEMBody.cpp:129-144
+    void build_global();               // FEMBody.cpp:90-109
+    void newton_dynamic();             // FEMBody.cpp:71-87, 112-126
+    void load_vector(Elem& e, MarkerView mv);  // FEMElement.cpp:27-69
+    void force_vector(Elem& e);        // FEMElement.cpp:72-97
+    void mass_stiff(Elem& e);          // FEMElement.cpp:100-138
+    static void transform(Elem& e);    // FEMElement.cpp:255-262
+    static void local_matrices(Elem& e);  // FEMElement.cpp:203-252
+    static void shape(const Elem& e, const double v[6], double zeta, double out[2]);  // FEMElement.cpp:141-160
+
+    FemScalars s_;
+    double L0_, angle0_, tCrit_;
+    int ndof_, nbc_, itNR_ = 0;
+    double resNR_ = 0.0, subRes_ = 0.0, subNum_ = 0.0, subDen_ = 0.0;
+    std::vector<Node> node_;
+    std::vector<Elem> el_;
+    std::vector<PosMap> pmap_;
+    BandMatrix M_, K_, W_;
+    std::vector<double> vec_, y_;
+};
+
+}  // namespace life

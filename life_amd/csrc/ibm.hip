@@ -1,0 +1,231 @@
+// Immersed-boundary kernels: support search, marker spacing, epsilon
+// matrix, interpolation + direct forcing, and the ordered spread.
+//
+// All arithmetic is FP64 with the reference's operation order
+// (IBMNode.cpp, Objects.cpp:235-321), for both population storages.
+// Compiled with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "d2q9_f64.h"
+#include "launch.h"
+
+namespace life {
+
+// IBMNodeClass::findSupport (IBMNode.cpp:139-179): lattice sites within 1.5
+// spacings of the marker in both directions, scanned i-major over the 5x5
+// block around the nearest site, kept in scan order (at most 9).
+__global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n) return;
+    if (flexible_only && m.body_flex[m.body[k]] != 0) return;
+    const double Dx = g.Dx;
+    const double x = m.pos[2 * k], y = m.pos[2 * k + 1];
+    const int inear = (int)round(x / Dx), jnear = (int)round(y / Dx);
+    int n = 0;
+    for (int i = inear - 2; i <= inear + 2; i++) {
+        const double dx = fabs(x / Dx - i);
+        for (int j = jnear - 2; j <= jnear + 2; j++) {
+            const double dy = fabs(y / Dx - j);
+            if (dx < 1.5 && dy < 1.5 && i >= 0 && i <= g.Nx - 1 && j >= 0 && j <= g.Ny - 1) {
+                if (n == kSupp) {
+                    *m.overflow = 1;
+                    continue;
+                }
+                m.supp_site[k * kSupp + n] = i * g.Ny + j;
+                m.supp_ij[(k * kSupp + n) * 2] = i;
+                m.supp_ij[(k * kSupp + n) * 2 + 1] = j;
+                m.supp_delta[k * kSupp + n] = dirac64(dx) * dirac64(dy);
+                n++;
+            }
+        }
+    }
+    for (int s = n; s < kSupp; s++) {
+        m.supp_site[k * kSupp + s] = -1;
+        m.supp_ij[(k * kSupp + s) * 2] = 0;
+        m.supp_ij[(k * kSupp + s) * 2 + 1] = 0;
+        m.supp_delta[k * kSupp + s] = 0.0;
+    }
+    m.supp_cnt[k] = n;
+}
+
+// IBMNodeClass::computeDs (IBMNode.cpp:182-204): distance (lattice units)
+// to the nearest other marker of the same body, starting from 10.
+__global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n) return;
+    const int b = m.body[k];
+    if (flexible_only && m.body_flex[b] != 0) return;
+    const double x = m.pos[2 * k], y = m.pos[2 * k + 1];
+    double ds = 10.0;
+    const int first = m.body_first[b], cnt = m.body_n[b];
+    for (int q = first; q < first + cnt; q++) {
+        if (q == k) continue;
+        const double dx = x - m.pos[2 * q], dy = y - m.pos[2 * q + 1];
+        const double mag = sqrt(0.0 + dx * dx + dy * dy) / g.Dx;
+        if (mag < ds) ds = mag;
+    }
+    m.ds[k] = ds;
+}
+
+// A_ij of ObjectsClass::computeEpsilon (Objects.cpp:271-300) for the system
+// of markers [first, first+n): sum over the supports s of marker i of
+// delta_i(s) * delta(x_j - s), times ds_j.
+__global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, double* __restrict__ A) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * n) return;
+    const int r = (int)(t / n), c = (int)(t - (long long)r * n);
+    const int mi = first + r, mj = first + c;
+    const double Dx = g.Dx;
+    const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
+    double a = 0.0;
+    const int cnt = m.supp_cnt[mi];
+    for (int s = 0; s < cnt; s++) {
+        const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
+        const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
+        a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
+    }
+    A[t] = a * m.ds[mj];
+}
+
+// ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
+// the supports (GridClass::macroscopic values: force_xy only) interpolated
+// with the support deltas, then F = 2 (Dt/Dx rho_I U - (rho u)_I).
+template <typename T>
+__global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n) return;
+    double iR = 0.0, mx = 0.0, my = 0.0;
+    const int cnt = m.supp_cnt[k];
+    for (int s = 0; s < cnt; s++) {
+        const long long id = m.supp_site[k * kSupp + s];
+        double fs[kNV], rho, ux, uy;
+        load_site(f, g.S, id, fs);
+        moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+        const double d = m.supp_delta[k * kSupp + s];
+        iR += rho * d * 1.0 * 1.0;
+        mx += rho * ux * d * 1.0 * 1.0;
+        my += rho * uy * d * 1.0 * 1.0;
+    }
+    const double vs = g.velScale;
+    m.force[2 * k] = 2.0 * (vs * iR * m.vel[2 * k] - mx);
+    m.force[2 * k + 1] = 2.0 * (vs * iR * m.vel[2 * k + 1] - my);
+}
+
+// ---------------------------------------------------------------- spread
+// The reference adds marker contributions to each site in marker order
+// (ORDERED, IBMNode.cpp:67-83).  Here every (marker, support) pair takes a
+// slot at its site, each touched site gets a compact index, the pairs are
+// scattered into per-site lists, and one thread per site sorts its list by
+// marker and sums it from zero: the same additions in the same order.
+
+__global__ void k_spread_clear(double* __restrict__ fibm, long long S, const int* __restrict__ uniq,
+                               const int* __restrict__ uniq_n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *uniq_n) return;
+    const long long id = uniq[k];
+    fibm[id] = 0.0;
+    fibm[S + id] = 0.0;
+}
+
+__global__ void k_spread_reset(int* uniq_n) { *uniq_n = 0; }
+
+__global__ void k_spread_count(MarkerDev m, SpreadScratch sc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.n * kSupp) return;
+    const int k = p / kSupp, s = p - k * kSupp;
+    if (s >= m.supp_cnt[k]) return;
+    const int site = m.supp_site[p];
+    const int slot = atomicAdd(&sc.site_count[site], 1);
+    sc.pair_slot[p] = slot;
+    if (slot == 0) {
+        const int u = atomicAdd(sc.uniq_n, 1);
+        sc.uniq[u] = site;
+        sc.site_uid[site] = u;
+    }
+}
+
+__global__ void k_spread_scatter(MarkerDev m, SpreadScratch sc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.n * kSupp) return;
+    const int k = p / kSupp, s = p - k * kSupp;
+    if (s >= m.supp_cnt[k]) return;
+    const int site = m.supp_site[p];
+    const int slot = sc.pair_slot[p];
+    if (slot >= kMaxPerSite) {
+        *sc.overflow = 1;
+        return;
+    }
+    sc.list[(long long)sc.site_uid[site] * kMaxPerSite + slot] = p;
+}
+
+__global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restrict__ fibm, long long S) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= *sc.uniq_n) return;
+    const int site = sc.uniq[u];
+    int n = sc.site_count[site];
+    if (n > kMaxPerSite) n = kMaxPerSite;
+    int pairs[kMaxPerSite];
+    for (int q = 0; q < n; q++) {
+        const int p = sc.list[(long long)u * kMaxPerSite + q];
+        int r = q - 1;
+        while (r >= 0 && pairs[r] > p) {
+            pairs[r + 1] = pairs[r];
+            r--;
+        }
+        pairs[r + 1] = p;
+    }
+    double fx = 0.0, fy = 0.0;
+    for (int q = 0; q < n; q++) {
+        const int p = pairs[q];
+        const int k = p / kSupp;
+        const double d = m.supp_delta[p];
+        const double e = m.eps[k], ds = m.ds[k];
+        fx += m.force[2 * k] * e * ds * 1.0 * d;
+        fy += m.force[2 * k + 1] * e * ds * 1.0 * d;
+    }
+    fibm[site] = fx;
+    fibm[S + site] = fy;
+    sc.site_count[site] = 0;
+}
+
+void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
+    if (m.n <= 0) return;
+    k_find_support<<<(m.n + 127) / 128, 128, 0, s>>>(m, g, flexible_only);
+}
+
+void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
+    if (m.n <= 0) return;
+    k_compute_ds<<<(m.n + 127) / 128, 128, 0, s>>>(m, g, flexible_only);
+}
+
+void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s) {
+    const long long t = (long long)n * n;
+    if (t <= 0) return;
+    k_epsilon_matrix<<<(unsigned)((t + 255) / 256), 256, 0, s>>>(m, g, first, n, A);
+}
+
+template <typename T>
+void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s) {
+    if (m.n <= 0) return;
+    k_interpolate<T><<<(m.n + 127) / 128, 128, 0, s>>>(f, m, g);
+}
+template void launch_interpolate<double>(const double*, const MarkerDev&, const GridScalars&, hipStream_t);
+template void launch_interpolate<float>(const float*, const MarkerDev&, const GridScalars&, hipStream_t);
+
+void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,
+                   hipStream_t s) {
+    const int np = m.n * kSupp;
+    const int cap = sc.cap;
+    if (dense_clear) {
+        launch_fill_f64(fibm, 0.0, g.S * 2, s);
+    } else if (cap > 0) {
+        k_spread_clear<<<(cap + 255) / 256, 256, 0, s>>>(fibm, g.S, sc.uniq, sc.uniq_n);
+    }
+    k_spread_reset<<<1, 1, 0, s>>>(sc.uniq_n);
+    if (np <= 0) return;
+    k_spread_count<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
+    k_spread_scatter<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
+    k_spread_reduce<<<(cap + 127) / 128, 128, 0, s>>>(m, sc, fibm, g.S);
+}
+
+}  // namespace life

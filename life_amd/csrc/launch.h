@@ -1,0 +1,92 @@
+// Kernel launchers shared between the HIP translation units and the host
+// runtime.  All launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "life_dev.h"
+
+namespace life {
+
+// FP64 plain method (lbm_f64.hip)
+void launch_push64(const double* src, double* dst, const GridScalars& g, const double* rho0, const double* u0,
+                   const double* fibm, bool explicit_state, hipStream_t s);
+void launch_support_push64(const double* src, double* dst, const GridScalars& g, const double* fibm,
+                           const int* sites, const int* count, int max_sites, hipStream_t s);
+
+// FP32-shifted populations (lbm_f32.hip)
+void launch_push32(const float* src, float* dst, const GridScalars& g, const double* rho0, const double* u0,
+                   const double* fibm, bool explicit_state, hipStream_t s);
+void launch_support_push32(const float* src, float* dst, const GridScalars& g, const double* fibm,
+                           const int* sites, const int* count, int max_sites, hipStream_t s);
+
+// Boundary conditions, outlet, initialisation and materialisation
+// (lbm_bc.hip), for T = double (plain) or float (shifted).
+struct BCArgs {
+    const BCSite* sites;
+    int nbc;
+    const double* u_in;   // Ny*2
+    const double* rho_in; // Ny
+    const double* delU;   // Ny*2
+    double ramp;
+};
+
+template <typename T>
+void launch_bc(const T* src, T* dst, const GridScalars& g, const BCArgs& a, const double* fibm, const double* rho0,
+               const double* u0, bool explicit_state, hipStream_t s);
+template <typename T>
+void launch_convective_speed(const T* src, const GridScalars& g, const double* fibm, const double* u0,
+                             bool explicit_state, double* ucol, double* delU, bool ordered, hipStream_t s);
+template <typename T>
+void launch_init_equilibrium(T* f, const GridScalars& g, const double* rho0, const double* u0, hipStream_t s);
+template <typename T>
+void launch_materialise(const T* f, const GridScalars& g, const double* fibm, const double* rho0, const double* u0,
+                        bool explicit_state, double* f_aos, double* rho, double* u, double* fibm_aos,
+                        hipStream_t s);
+template <typename T>
+void launch_upload(T* f, const GridScalars& g, const double* f_aos, hipStream_t s);
+void launch_init_moments(double* rho0, double* u0, const GridScalars& g, int mode, double ux, double uy,
+                         const double* u_in, const BCSite* bc, int nbc, hipStream_t s);
+void launch_fibm_from_aos(double* fibm, const GridScalars& g, const double* fibm_aos, hipStream_t s);
+void launch_fill_f64(double* p, double v, long long n, hipStream_t s);
+void launch_fill_i32(int* p, int v, long long n, hipStream_t s);
+
+// IBM (ibm.hip)
+struct MarkerDev {
+    int n;
+    double* pos;       // n*2 (physical)
+    double* vel;       // n*2 (physical)
+    double* force;     // n*2 (lattice)
+    double* eps;       // n
+    double* ds;        // n
+    int* body;         // n
+    const int* body_first;
+    const int* body_n;
+    const int* body_flex;
+    int* supp_cnt;     // n
+    int* supp_site;    // n*9, lattice id or -1
+    int* supp_ij;      // n*9*2
+    double* supp_delta;// n*9
+    int* overflow;     // 1: set when a marker finds more than 9 supports
+};
+
+struct SpreadScratch {
+    int* site_count;   // N, zero between spreads
+    int* site_uid;     // N
+    int* uniq;         // cap, lattice ids of support sites of the last spread
+    int* uniq_n;       // 1
+    int* pair_slot;    // n*9
+    int* list;         // cap*kMaxPerSite
+    int* overflow;     // 1
+    int cap;
+};
+
+void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s);
+void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s);
+void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s);
+template <typename T>
+void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s);
+void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,
+                   hipStream_t s);
+
+}  // namespace life

@@ -1,0 +1,206 @@
+// FP32-shifted populations: fused push stream-collide kernels.
+//
+// Stored value g_v = f_v - w_v (rho0 = 1) in FP32, so that the O(1) part of
+// each population never enters the rounding.  Moments and collision run in
+// FP32 on the deviations:
+//   drho = sum g_v, rho = 1 + drho, j = sum c_v g_v, u = (j + F/2) / rho
+//   BGK:  feq_v - w_v = w_v (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u))
+//   CM:   f*_v - w_v = (feqCM_v(1,u) - w_v) + drho feqCM_v(1,u)
+//                      + sum_m Minv_vm(u) k_m over the non-equilibrium
+//                      central moments m = 1,2,4,5,6,7 (Grid.cpp:125-223)
+// Same push/wrap pattern and same support-site treatment as the FP64 path
+// (lbm_f64.hip).  72 B of HBM traffic per site update.
+#include <hip/hip_runtime.h>
+
+#include "dev_util.h"
+#include "launch.h"
+
+namespace life {
+
+#define LIFE_DF __device__ __forceinline__
+
+LIFE_DF float w32(int v) { return v == 0 ? 4.0f / 9.0f : (v < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
+
+// BGK + Guo on deviations.
+LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rho, float ux, float uy, float Fx,
+                           float Fy, float go[kNV]) {
+    const float usq = 1.5f * (ux * ux + uy * uy);
+    const float cF = 1.0f - 0.5f * omega;
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        const float cx = (float)kCX[v], cy = (float)kCY[v];
+        const float cu = cx * ux + cy * uy;
+        const float eq = w32(v) * (drho + rho * (3.0f * cu + 4.5f * cu * cu - usq));
+        const float fv = 3.0f * w32(v) * ((cx - ux) * Fx + (cy - uy) * Fy + 3.0f * cu * (cx * Fx + cy * Fy));
+        go[v] = g[v] + omega * (eq - g[v]) + cF * fv;
+    }
+}
+
+// Central moments on deviations (De Rosis 2019 basis, Grid.cpp:106-223).
+LIFE_DF void collide_cm32(float omega, float cs2, float cs4, const float g[kNV], float drho, float ux, float uy,
+                          float Fx, float Fy, float go[kNV]) {
+    float k4p = ux * ux - uy * uy, k5p = ux * uy;  // sum_v w_v (...) in closed form
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        const float dx = (float)kCX[v] - ux, dy = (float)kCY[v] - uy;
+        k4p += g[v] * (dx * dx - dy * dy);
+        k5p += g[v] * dx * dy;
+    }
+    const float k1 = 0.5f * Fx, k2 = 0.5f * Fy;
+    const float k4 = (1.0f - omega) * k4p, k5 = (1.0f - omega) * k5p;
+    const float k6 = 0.5f * Fy * cs2, k7 = 0.5f * Fx * cs2;
+    const float X = ux, Y = uy, XX = ux * ux, YY = uy * uy, XY = ux * uy;
+    const float XYY = X * YY, XXY = XX * Y;
+    // M^-1 columns for k1, k2, k4, k5, k6, k7
+    const float m1[kNV] = {2.0f * XYY - 2.0f * X,
+                           X - XYY - 0.5f * YY + 0.5f,
+                           X - XYY + 0.5f * YY - 0.5f,
+                           -XYY - XY,
+                           -XYY + XY,
+                           0.25f * Y + 0.5f * XY + 0.5f * XYY + 0.25f * YY,
+                           0.25f * Y - 0.5f * XY + 0.5f * XYY - 0.25f * YY,
+                           0.5f * XYY - 0.5f * XY - 0.25f * Y + 0.25f * YY,
+                           0.5f * XY - 0.25f * Y + 0.5f * XYY - 0.25f * YY};
+    const float m2[kNV] = {2.0f * XXY - 2.0f * Y,
+                           -XXY - XY,
+                           -XXY + XY,
+                           Y - XXY - 0.5f * XX + 0.5f,
+                           Y - XXY + 0.5f * XX - 0.5f,
+                           0.25f * X + 0.5f * XY + 0.5f * XXY + 0.25f * XX,
+                           0.25f * X - 0.5f * XY + 0.5f * XXY - 0.25f * XX,
+                           0.5f * XY - 0.25f * X + 0.5f * XXY - 0.25f * XX,
+                           0.5f * XXY - 0.5f * XY - 0.25f * X + 0.25f * XX};
+    const float m4[kNV] = {0.5f * YY - 0.5f * XX,
+                           0.25f * XX + 0.25f * X - 0.25f * YY + 0.25f,
+                           0.25f * XX - 0.25f * X - 0.25f * YY + 0.25f,
+                           0.25f * XX - 0.25f * YY - 0.25f * Y - 0.25f,
+                           0.25f * XX - 0.25f * YY + 0.25f * Y - 0.25f,
+                           -0.125f * XX - 0.125f * X + 0.125f * YY + 0.125f * Y,
+                           -0.125f * XX + 0.125f * X + 0.125f * YY - 0.125f * Y,
+                           -0.125f * XX - 0.125f * X + 0.125f * YY - 0.125f * Y,
+                           -0.125f * XX + 0.125f * X + 0.125f * YY + 0.125f * Y};
+    const float m5[kNV] = {4.0f * XY,
+                           -Y - 2.0f * XY,
+                           Y - 2.0f * XY,
+                           -X - 2.0f * XY,
+                           X - 2.0f * XY,
+                           0.5f * X + 0.5f * Y + XY + 0.25f,
+                           XY - 0.5f * Y - 0.5f * X + 0.25f,
+                           0.5f * Y - 0.5f * X + XY - 0.25f,
+                           0.5f * X - 0.5f * Y + XY - 0.25f};
+    const float m6[kNV] = {2.0f * Y, -Y, -Y, -Y - 0.5f, 0.5f - Y,
+                           0.5f * Y + 0.25f, 0.5f * Y - 0.25f, 0.5f * Y - 0.25f, 0.5f * Y + 0.25f};
+    const float m7[kNV] = {2.0f * X, -X - 0.5f, 0.5f - X, -X, -X,
+                           0.5f * X + 0.25f, 0.5f * X - 0.25f, 0.5f * X + 0.25f, 0.5f * X - 0.25f};
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        const float cx = (float)kCX[v], cy = (float)kCY[v];
+        const float bx = 9.0f * (cx * cx) * XX + 6.0f * cx * X - 3.0f * XX;
+        const float by = 9.0f * (cy * cy) * YY + 6.0f * cy * Y - 3.0f * YY;
+        const float dev = w32(v) * (0.5f * (bx + by) + 0.25f * bx * by);  // feqCM(1,u) - w
+        const float feq1 = w32(v) + dev;
+        go[v] = dev + drho * feq1 + m1[v] * k1 + m2[v] * k2 + m4[v] * k4 + m5[v] * k5 + m6[v] * k6 + m7[v] * k7;
+    }
+}
+
+template <bool CM>
+LIFE_DF void collide32(const GridScalars& g, const float gs[kNV], float drho, float ux, float uy, float Fx, float Fy,
+                       float go[kNV]) {
+    if (CM) collide_cm32((float)g.omega, (float)g.cs2, (float)g.cs4, gs, drho, ux, uy, Fx, Fy, go);
+    else collide_bgk32((float)g.omega, gs, drho, 1.0f + drho, ux, uy, Fx, Fy, go);
+}
+
+LIFE_DF void moments32(const float gs[kNV], float Fx, float Fy, float& drho, float& ux, float& uy) {
+    const float dr = ((gs[0] + gs[1]) + (gs[2] + gs[3])) + ((gs[4] + gs[5]) + (gs[6] + gs[7])) + gs[8];
+    const float jx = (gs[1] - gs[2]) + (gs[5] - gs[6]) + (gs[7] - gs[8]);
+    const float jy = (gs[3] - gs[4]) + (gs[5] - gs[6]) + (gs[8] - gs[7]);
+    const float inv = 1.0f / (1.0f + dr);
+    drho = dr;
+    ux = (jx + 0.5f * Fx) * inv;
+    uy = (jy + 0.5f * Fy) * inv;
+}
+
+template <bool CM, bool EXPLICIT>
+__global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, float* __restrict__ dst, GridScalars g,
+                                                const double* __restrict__ rho0, const double* __restrict__ u0,
+                                                const double* __restrict__ fibm, unsigned nblocks) {
+    const long long id = (long long)xcd_block(blockIdx.x, nblocks) * blockDim.x + threadIdx.x;
+    if (id >= g.N) return;
+    const long long S = g.S;
+    const int Ny = g.Ny, Nx = g.Nx;
+    const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
+    float gs[kNV], go[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++) gs[v] = __builtin_nontemporal_load(&src[v * S + id]);
+    float drho, ux, uy, Fx = (float)g.Fx, Fy = (float)g.Fy;
+    if (EXPLICIT) {
+        drho = (float)(rho0[id] - 1.0);
+        ux = (float)u0[2 * id];
+        uy = (float)u0[2 * id + 1];
+        Fx = (float)(g.Fx + fibm[id]);
+        Fy = (float)(g.Fy + fibm[S + id]);
+    } else {
+        moments32(gs, Fx, Fy, drho, ux, uy);
+    }
+    collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
+    const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? Ny - 1 : j - 1, jp = j == Ny - 1 ? 0 : j + 1;
+    const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
+    __builtin_nontemporal_store(go[0], &dst[0 * S + r0 + j]);
+    __builtin_nontemporal_store(go[1], &dst[1 * S + rp + j]);
+    __builtin_nontemporal_store(go[2], &dst[2 * S + rm + j]);
+    dst[3 * S + r0 + jp] = go[3];
+    dst[4 * S + r0 + jm] = go[4];
+    dst[5 * S + rp + jp] = go[5];
+    dst[6 * S + rm + jm] = go[6];
+    dst[7 * S + rp + jm] = go[7];
+    dst[8 * S + rm + jp] = go[8];
+}
+
+template <bool CM>
+__global__ __launch_bounds__(256) void k_support_push32(const float* __restrict__ src, float* __restrict__ dst,
+                                                        GridScalars g, const double* __restrict__ fibm,
+                                                        const int* __restrict__ sites,
+                                                        const int* __restrict__ count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *count) return;
+    const long long S = g.S, id = sites[k];
+    const int Ny = g.Ny, Nx = g.Nx;
+    const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
+    float gs[kNV], go[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++) gs[v] = src[v * S + id];
+    const float Fx = (float)(g.Fx + fibm[id]), Fy = (float)(g.Fy + fibm[S + id]);
+    float drho, ux, uy;
+    moments32(gs, Fx, Fy, drho, ux, uy);
+    collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        int ii = i + kCX[v], jj = j + kCY[v];
+        ii = ii < 0 ? Nx - 1 : (ii >= Nx ? 0 : ii);
+        jj = jj < 0 ? Ny - 1 : (jj >= Ny ? 0 : jj);
+        dst[v * S + (long long)ii * Ny + jj] = go[v];
+    }
+}
+
+void launch_push32(const float* src, float* dst, const GridScalars& g, const double* rho0, const double* u0,
+                   const double* fibm, bool explicit_state, hipStream_t s) {
+    const unsigned nb = (unsigned)((g.N + 255) / 256);
+    if (g.cm) {
+        if (explicit_state) k_push32<true, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+        else k_push32<true, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    } else {
+        if (explicit_state) k_push32<false, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+        else k_push32<false, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    }
+}
+
+void launch_support_push32(const float* src, float* dst, const GridScalars& g, const double* fibm, const int* sites,
+                           const int* count, int max_sites, hipStream_t s) {
+    if (max_sites <= 0) return;
+    const unsigned nb = (unsigned)((max_sites + 255) / 256);
+    if (g.cm) k_support_push32<true><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+    else k_support_push32<false><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+}
+
+}  // namespace life

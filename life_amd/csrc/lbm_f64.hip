@@ -1,0 +1,108 @@
+// Plain method (FP64 populations): fused push stream-collide kernels.
+//
+// Reference step (Grid.cpp:36-100): every site collides with the rho/u/F
+// stored at the end of the previous step and pushes its nine post-collision
+// populations to its neighbours with periodic wrap (Grid.cpp:226-244).  The
+// stored rho/u are the moments of the stored populations, with force_xy
+// (Grid.cpp:282-299) or, at sites that received an IBM force,
+// force_xy + force_ibm (IBMNode.cpp:97-136).  This kernel therefore reads
+// only f (72 B/site) and recomputes the moments; the few support sites of
+// the last spread are redone by k_support_push with their force_ibm.
+//
+// Compiled with -ffp-contract=off: bit-compatible with the reference.
+#include <hip/hip_runtime.h>
+
+#include "d2q9_f64.h"
+#include "dev_util.h"
+#include "launch.h"
+
+namespace life {
+
+template <bool CM, bool EXPLICIT>
+__global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, double* __restrict__ dst,
+                                                GridScalars g, const double* __restrict__ rho0,
+                                                const double* __restrict__ u0, const double* __restrict__ fibm,
+                                                unsigned nblocks) {
+    const long long id = (long long)xcd_block(blockIdx.x, nblocks) * blockDim.x + threadIdx.x;
+    if (id >= g.N) return;
+    const long long S = g.S;
+    const int Ny = g.Ny, Nx = g.Nx;
+    const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
+    double fs[kNV], fo[kNV];
+    load_site(src, S, id, fs);
+    double rho, ux, uy, Fx = g.Fx0, Fy = g.Fy0;
+    if (EXPLICIT) {
+        // rho/u/force_ibm exactly as stored by initialisation or a restart
+        rho = rho0[id];
+        ux = u0[2 * id];
+        uy = u0[2 * id + 1];
+        Fx = g.Fx + fibm[id];
+        Fy = g.Fy + fibm[S + id];
+    } else {
+        moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+    }
+    collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
+    const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? Ny - 1 : j - 1, jp = j == Ny - 1 ? 0 : j + 1;
+    const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
+    dst[0 * S + r0 + j] = fo[0];
+    dst[1 * S + rp + j] = fo[1];
+    dst[2 * S + rm + j] = fo[2];
+    dst[3 * S + r0 + jp] = fo[3];
+    dst[4 * S + r0 + jm] = fo[4];
+    dst[5 * S + rp + jp] = fo[5];
+    dst[6 * S + rm + jm] = fo[6];
+    dst[7 * S + rp + jm] = fo[7];
+    dst[8 * S + rm + jp] = fo[8];
+}
+
+// Redo the push of the support sites of the last spread, where the stored
+// u includes force_ibm and the Guo/central-moment forcing uses
+// force_xy + force_ibm.  sites[0..*count) are lattice ids.
+template <bool CM>
+__global__ __launch_bounds__(256) void k_support_push64(const double* __restrict__ src, double* __restrict__ dst,
+                                                        GridScalars g, const double* __restrict__ fibm,
+                                                        const int* __restrict__ sites,
+                                                        const int* __restrict__ count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *count) return;
+    const long long S = g.S;
+    const long long id = sites[k];
+    const int Ny = g.Ny, Nx = g.Nx;
+    const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
+    double fs[kNV], fo[kNV];
+    load_site(src, S, id, fs);
+    const double Fx = g.Fx + fibm[id], Fy = g.Fy + fibm[S + id];
+    double rho, ux, uy;
+    moments64(fs, Fx, Fy, rho, ux, uy);
+    collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        int ii = i + kCX[v], jj = j + kCY[v];
+        ii = ii < 0 ? Nx - 1 : (ii >= Nx ? 0 : ii);
+        jj = jj < 0 ? Ny - 1 : (jj >= Ny ? 0 : jj);
+        dst[v * S + (long long)ii * Ny + jj] = fo[v];
+    }
+}
+
+void launch_push64(const double* src, double* dst, const GridScalars& g, const double* rho0, const double* u0,
+                   const double* fibm, bool explicit_state, hipStream_t s) {
+    const unsigned nb = (unsigned)((g.N + 255) / 256);
+    if (g.cm) {
+        if (explicit_state) k_push64<true, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+        else k_push64<true, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    } else {
+        if (explicit_state) k_push64<false, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+        else k_push64<false, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    }
+}
+
+void launch_support_push64(const double* src, double* dst, const GridScalars& g, const double* fibm,
+                           const int* sites, const int* count, int max_sites, hipStream_t s) {
+    if (max_sites <= 0) return;
+    const unsigned nb = (unsigned)((max_sites + 255) / 256);
+    if (g.cm) k_support_push64<true><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+    else k_support_push64<false><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+}
+
+}  // namespace life

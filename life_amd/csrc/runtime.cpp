@@ -1,0 +1,603 @@
+// Device lattice runtime and the layer-1 C ABI (life_grid_*, life_lbm_*,
+// life_ibm_*).  Compiled with -ffp-contract=off: the derived scalars, the
+// inlet profile and the ramp follow the reference's rounding.
+#include "runtime.h"
+
+#include "lu.h"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace life {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+CaseScalars derive_scalars(const life_params& p) {
+    CaseScalars s{};
+    const double rho0 = 1.0;
+    s.cs = 1.0 / std::sqrt(3.0);
+    s.cs2 = s.cs * s.cs;
+    s.cs4 = s.cs * s.cs * s.cs * s.cs;
+    if (p.omega > 0.0) {
+        s.omega = p.omega;
+    } else {
+        const double c = 1.0 / std::sqrt(3.0);
+        const double h = p.height_p / (p.Ny - 1);
+        s.omega = 1.0 / (p.nu_p * p.tStep / ((c * c) * (h * h)) + 0.5);
+    }
+    s.tau = 1.0 / s.omega;
+    s.nu = (s.tau - 0.5) * s.cs2;
+    s.Dx = p.height_p / (p.Ny - 1);
+    s.Dt = (s.Dx * s.Dx) * s.nu / p.nu_p;
+    s.Dm = (p.rho_p / rho0) * (s.Dx * s.Dx * s.Dx);
+    s.Drho = p.rho_p / rho0;
+    return s;
+}
+
+double ramp_coefficient(const life_params& p, double Dt, int t) {
+    if (p.inlet_ramp > 0.0 && Dt * t <= p.inlet_ramp) return (1.0 - std::cos(M_PI * Dt * t / p.inlet_ramp)) / 2.0;
+    return 1.0;
+}
+
+namespace {
+
+// Site type of the reference's type map (Grid.cpp:931-957): bottom/top
+// override left/right at the corners.
+int site_type(const life_params& p, int i, int j) {
+    int t = LIFE_FLUID;
+    if (i == 0) t = p.wall_left;
+    else if (i == p.Nx - 1) t = p.wall_right;
+    if (j == 0) t = p.wall_bottom;
+    else if (j == p.Ny - 1) t = p.wall_top;
+    return t;
+}
+
+// GridClass::getNormalVector, Grid.cpp:498-545.
+void normal_vector(const life_params& p, int i, int j, int& nx, int& ny, int& nd) {
+    nx = ny = 0;
+    nd = 0;
+    if (i == 0) {
+        nx = 1;
+        nd = 0;
+    } else if (i == p.Nx - 1) {
+        nx = -1;
+        nd = 0;
+    }
+    if (j == 0) {
+        ny = 1;
+        nd = 1;
+    } else if (j == p.Ny - 1) {
+        ny = -1;
+        nd = 1;
+    }
+    if (nx != 0 && ny != 0) {
+        const int tx = site_type(p, i + nx, j), ty = site_type(p, i, j + ny);
+        if (tx == LIFE_FLUID && ty == LIFE_FLUID)
+            throw std::runtime_error("Corner node is surrounded by fluid lattice sites");
+        else if (tx == LIFE_FLUID) {
+            nd = 0;
+            ny = 0;
+        } else if (ty == LIFE_FLUID) {
+            nd = 1;
+            nx = 0;
+        }
+    }
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return 2;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+}  // namespace
+
+void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    const double ramp = ramp_coefficient(g->p, g->cs.Dt, t);
+    const int src = g->cur, dst = 1 - g->cur;
+    BCArgs a{g->bc, g->nbc, g->u_in, g->rho_in, g->delU, ramp};
+    const bool ex = g->explicit_state;
+    hipStream_t s = g->stream;
+    if (g->prec == kFP64) {
+        const double* fs = static_cast<const double*>(g->f[src]);
+        double* fd = static_cast<double*>(g->f[dst]);
+        if (g->convective) launch_convective_speed<double>(fs, g->g, g->fibm, g->u0, ex, g->ucol, g->delU, true, s);
+        if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+        launch_push64(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
+        if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        if (!ex && g->md.n > 0) launch_support_push64(fs, fd, g->g, g->fibm, g->sc.uniq, g->sc.uniq_n, g->sc.cap, s);
+        launch_bc<double>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
+    } else {
+        const float* fs = static_cast<const float*>(g->f[src]);
+        float* fd = static_cast<float*>(g->f[dst]);
+        if (g->convective) launch_convective_speed<float>(fs, g->g, g->fibm, g->u0, ex, g->ucol, g->delU, false, s);
+        if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+        launch_push32(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
+        if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        if (!ex && g->md.n > 0) launch_support_push32(fs, fd, g->g, g->fibm, g->sc.uniq, g->sc.uniq_n, g->sc.cap, s);
+        launch_bc<float>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
+    }
+    LIFE_HIP(hipGetLastError());
+    g->cur = dst;
+    g->explicit_state = false;
+}
+
+void ibm_interpolate(life_grid* g) {
+    if (g->md.n <= 0) return;
+    if (g->prec == kFP64) launch_interpolate<double>(static_cast<const double*>(g->f[g->cur]), g->md, g->g, g->stream);
+    else launch_interpolate<float>(static_cast<const float*>(g->f[g->cur]), g->md, g->g, g->stream);
+    LIFE_HIP(hipGetLastError());
+}
+
+void ibm_spread(life_grid* g) {
+    launch_spread(g->md, g->g, g->fibm, g->sc, g->dense_clear_next, g->stream);
+    g->dense_clear_next = false;
+    LIFE_HIP(hipGetLastError());
+}
+
+void ibm_find_support(life_grid* g, int flexible_only) {
+    if (g->md.n <= 0) return;
+    launch_find_support(g->md, g->g, flexible_only, g->stream);
+    launch_compute_ds(g->md, g->g, flexible_only, g->stream);
+    LIFE_HIP(hipGetLastError());
+}
+
+void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host) {
+    const size_t need = (size_t)n * n;
+    if (need > g->epsA_cap) {
+        dfree(g->epsA);
+        g->epsA = dalloc<double>(need);
+        g->epsA_cap = need;
+    }
+    launch_epsilon_matrix(g->md, g->g, first, n, g->epsA, g->stream);
+    LIFE_HIP(hipGetLastError());
+    LIFE_HIP(hipMemcpyAsync(A_host, g->epsA, need * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+}
+
+void check_flags(life_grid* g) {
+    if (!g->flags) return;
+    int h[2] = {0, 0};
+    LIFE_HIP(hipMemcpyAsync(h, g->flags, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    if (h[0]) throw std::runtime_error("Support buffer size is not big enough for number of support points");
+    if (h[1]) throw std::runtime_error("more than 64 markers share one support site");
+}
+
+static void free_markers(life_grid* g) {
+    MarkerDev& m = g->md;
+    for (void* p : {(void*)m.pos, (void*)m.vel, (void*)m.force, (void*)m.eps, (void*)m.ds, (void*)m.body,
+                    (void*)m.supp_cnt, (void*)m.supp_site, (void*)m.supp_ij, (void*)m.supp_delta,
+                    (void*)g->body_first, (void*)g->body_n, (void*)g->body_flex, (void*)g->sc.site_count,
+                    (void*)g->sc.site_uid, (void*)g->sc.uniq, (void*)g->sc.uniq_n, (void*)g->sc.pair_slot,
+                    (void*)g->sc.list, (void*)g->flags})
+        dfree(p);
+    m = MarkerDev{};
+    g->sc = SpreadScratch{};
+    g->body_first = g->body_n = g->body_flex = nullptr;
+    g->flags = nullptr;
+}
+
+}  // namespace life
+
+using namespace life;
+
+extern "C" {
+
+const char* life_last_error(void) { return life::g_error.c_str(); }
+
+const char* life_version(void) { return "life_hip 0.1 (gfx950)"; }
+
+int life_device_count(int* count) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int life_grid_create(const life_params* p, int precision, int device, life_grid** out) {
+    *out = nullptr;
+    life_grid* g = new life_grid();
+    int rc = guarded([&] {
+        if (p->Nx < 4 || p->Ny < 4) throw std::runtime_error("grid must be at least 4x4");
+        if (precision != LIFE_FP64 && precision != LIFE_FP32_SHIFTED) throw std::runtime_error("unknown precision");
+        if (p->wall_left == LIFE_CONVECTIVE || p->wall_bottom == LIFE_CONVECTIVE || p->wall_top == LIFE_CONVECTIVE)
+            throw std::runtime_error("Currently convective BC only supported for right boundary");
+        if ((long long)p->Nx * p->Ny > (1LL << 31) - 1) throw std::runtime_error("lattice too large for 32-bit site ids");
+        g->device = device;
+        LIFE_HIP(hipSetDevice(device));
+        LIFE_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        LIFE_HIP(hipEventCreate(&g->ev_a));
+        LIFE_HIP(hipEventCreate(&g->ev_b));
+        LIFE_HIP(hipEventCreate(&g->ev_k0));
+        LIFE_HIP(hipEventCreate(&g->ev_k1));
+        g->p = *p;
+        g->prec = precision;
+        g->cs = derive_scalars(*p);
+        const int Nx = p->Nx, Ny = p->Ny;
+        GridScalars& s = g->g;
+        s.Nx = Nx;
+        s.Ny = Ny;
+        s.N = (long long)Nx * Ny;
+        s.S = (s.N + 255) / 256 * 256;
+        s.cm = p->central_moments ? 1 : 0;
+        s.omega = g->cs.omega;
+        s.cs2 = g->cs.cs2;
+        s.cs4 = g->cs.cs4;
+        const double rho_init = 1.0;
+        const double DxDt = g->cs.Dx * g->cs.Dt;
+        s.Fx = (rho_init * g->cs.Drho * p->gravityX + p->dpdx) * (DxDt * DxDt) / g->cs.Dm;
+        s.Fy = (rho_init * g->cs.Drho * p->gravityY + p->dpdy) * (DxDt * DxDt) / g->cs.Dm;
+        s.Fx0 = s.Fx + 0.0;
+        s.Fy0 = s.Fy + 0.0;
+        s.Dx = g->cs.Dx;
+        s.velScale = g->cs.Dt / g->cs.Dx;
+
+        // boundary sites in the reference's i-major order (Grid.cpp:931-957)
+        for (int i = 0; i < Nx; i++) {
+            const bool edge_i = (i == 0 || i == Nx - 1);
+            for (int j = 0; j < Ny; j++) {
+                if (!edge_i && j != 0 && j != Ny - 1) continue;
+                const int t = site_type(*p, i, j);
+                if (t == LIFE_FLUID) continue;
+                int nx, ny, nd;
+                normal_vector(*p, i, j, nx, ny, nd);
+                BCSite b{};
+                b.id = i * Ny + j;
+                b.type = (int8_t)t;
+                b.nx = (int8_t)nx;
+                b.ny = (int8_t)ny;
+                b.nd = (int8_t)nd;
+                b.i = i;
+                b.j = j;
+                g->bc_host.push_back(b);
+                if (t == LIFE_CONVECTIVE) g->convective = true;
+            }
+        }
+        g->nbc = (int)g->bc_host.size();
+        // inlet profile (Grid.cpp:959-1001)
+        const double Dx = g->cs.Dx, Dt = g->cs.Dt;
+        g->u_in_host.assign((size_t)Ny * 2, 0.0);
+        g->rho_in_host.assign(Ny, 1.0);
+        for (int j = 0; j < Ny; j++) {
+            double* ui = &g->u_in_host[2 * j];
+            if (p->profile == LIFE_UNIFORM) {
+                ui[0] = p->uxInlet_p * Dt / Dx;
+                ui[1] = p->uyInlet_p * Dt / Dx;
+            } else if (p->profile == LIFE_PARABOLIC) {
+                const double R = p->height_p / 2.0, Y = j * Dx - R;
+                ui[0] = 1.5 * (p->uxInlet_p * Dt / Dx) * (1.0 - (Y / R) * (Y / R));
+                ui[1] = 1.5 * (p->uyInlet_p * Dt / Dx) * (1.0 - (Y / R) * (Y / R));
+            } else if (p->profile == LIFE_SHEAR) {
+                const double H = p->height_p, Y = j * Dx;
+                ui[0] = (p->uxInlet_p * Dt / Dx) * (Y / H);
+                ui[1] = (p->uyInlet_p * Dt / Dx) * (Y / H);
+            } else if (p->profile == LIFE_BOUNDARY_LAYER) {
+                const double H = p->height_p, Y = j * Dx;
+                ui[0] = ((1.5 * p->uxInlet_p * Dt / Dx) / (H * H)) * Y * (2.0 * H - Y);
+                ui[1] = ((1.5 * p->uyInlet_p * Dt / Dx) / (H * H)) * Y * (2.0 * H - Y);
+            } else {
+                throw std::runtime_error("unknown inlet profile");
+            }
+        }
+        // device buffers
+        const size_t S = (size_t)s.S;
+        for (int b = 0; b < 2; b++)
+            g->f[b] = precision == LIFE_FP64 ? (void*)dalloc<double>(9 * S) : (void*)dalloc<float>(9 * S);
+        g->rho0 = dalloc<double>(S);
+        g->u0 = dalloc<double>(2 * S);
+        g->fibm = dalloc<double>(2 * S);
+        g->bc = dalloc<BCSite>(g->nbc);
+        g->u_in = dalloc<double>(2 * Ny);
+        g->rho_in = dalloc<double>(Ny);
+        g->delU = dalloc<double>(2 * Ny);
+        g->ucol = dalloc<double>(6 * Ny);
+        hipStream_t st = g->stream;
+        if (g->nbc)
+            LIFE_HIP(hipMemcpyAsync(g->bc, g->bc_host.data(), sizeof(BCSite) * g->nbc, hipMemcpyHostToDevice, st));
+        LIFE_HIP(hipMemcpyAsync(g->u_in, g->u_in_host.data(), sizeof(double) * 2 * Ny, hipMemcpyHostToDevice, st));
+        LIFE_HIP(hipMemcpyAsync(g->rho_in, g->rho_in_host.data(), sizeof(double) * Ny, hipMemcpyHostToDevice, st));
+        launch_fill_f64(g->delU, 0.0, 2 * Ny, st);
+        launch_fill_f64(g->fibm, 0.0, 2 * (long long)S, st);
+        // initial moments (Grid.cpp:1004-1033) and f = f_eq (Grid.cpp:1048-1061)
+        int mode = 0;
+        double ux0 = 0.0, uy0 = 0.0;
+        if (p->inlet_ramp > 0.0) {
+            mode = 0;
+        } else if (p->profile != LIFE_UNIFORM) {
+            mode = 1;
+        } else {
+            ux0 = p->ux0_p * Dt / Dx;
+            uy0 = p->uy0_p * Dt / Dx;
+        }
+        launch_init_moments(g->rho0, g->u0, s, mode, ux0, uy0, g->u_in, g->bc, g->nbc, st);
+        if (precision == LIFE_FP64) launch_init_equilibrium<double>(static_cast<double*>(g->f[0]), s, g->rho0, g->u0, st);
+        else launch_init_equilibrium<float>(static_cast<float*>(g->f[0]), s, g->rho0, g->u0, st);
+        LIFE_HIP(hipGetLastError());
+        LIFE_HIP(hipStreamSynchronize(st));
+        g->cur = 0;
+        g->explicit_state = true;
+    });
+    if (rc) {
+        life_grid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return 0;
+}
+
+int life_grid_destroy(life_grid* g) {
+    if (!g) return 0;
+    return guarded([&] {
+        (void)hipSetDevice(g->device);
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        free_markers(g);
+        for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm, (void*)g->bc,
+                        (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA})
+            dfree(p);
+        for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1})
+            if (e) (void)hipEventDestroy(e);
+        if (g->stream) (void)hipStreamDestroy(g->stream);
+        delete g;
+    });
+}
+
+int life_grid_scalars(const life_grid* g, double out[10]) {
+    const CaseScalars& s = g->cs;
+    const double v[10] = {s.omega, s.tau, s.nu, s.Dx, s.Dt, s.Dm, s.Drho, s.cs, s.cs2, s.cs4};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+
+int life_lbm_kernel(life_grid* g, int t) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        lbm_step(g, t, nullptr, nullptr);
+    });
+}
+
+int life_grid_sync(life_grid* g) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int life_grid_download(life_grid* g, double* f_aos, double* rho, double* u, double* force_ibm) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        const size_t N = (size_t)g->g.N;
+        double *df = nullptr, *dr = nullptr, *du = nullptr, *db = nullptr;
+        try {
+            if (f_aos) df = dalloc<double>(9 * N);
+            if (rho) dr = dalloc<double>(N);
+            if (u) du = dalloc<double>(2 * N);
+            if (force_ibm) db = dalloc<double>(2 * N);
+            if (g->prec == kFP64)
+                launch_materialise<double>(static_cast<const double*>(g->f[g->cur]), g->g, g->fibm, g->rho0, g->u0,
+                                           g->explicit_state, df, dr, du, db, g->stream);
+            else
+                launch_materialise<float>(static_cast<const float*>(g->f[g->cur]), g->g, g->fibm, g->rho0, g->u0,
+                                          g->explicit_state, df, dr, du, db, g->stream);
+            LIFE_HIP(hipGetLastError());
+            if (df) LIFE_HIP(hipMemcpyAsync(f_aos, df, 9 * N * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+            if (dr) LIFE_HIP(hipMemcpyAsync(rho, dr, N * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+            if (du) LIFE_HIP(hipMemcpyAsync(u, du, 2 * N * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+            if (db) LIFE_HIP(hipMemcpyAsync(force_ibm, db, 2 * N * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+            LIFE_HIP(hipStreamSynchronize(g->stream));
+        } catch (...) {
+            dfree(df), dfree(dr), dfree(du), dfree(db);
+            throw;
+        }
+        dfree(df), dfree(dr), dfree(du), dfree(db);
+    });
+}
+
+int life_grid_upload(life_grid* g, const double* f_aos, const double* rho, const double* u, const double* force_ibm) {
+    return guarded([&] {
+        if (!f_aos || !rho || !u) throw std::runtime_error("life_grid_upload needs f, rho and u");
+        LIFE_HIP(hipSetDevice(g->device));
+        const size_t N = (size_t)g->g.N;
+        double* df = dalloc<double>(9 * N);
+        double* db = nullptr;
+        try {
+            LIFE_HIP(hipMemcpyAsync(df, f_aos, 9 * N * sizeof(double), hipMemcpyHostToDevice, g->stream));
+            LIFE_HIP(hipMemcpyAsync(g->rho0, rho, N * sizeof(double), hipMemcpyHostToDevice, g->stream));
+            LIFE_HIP(hipMemcpyAsync(g->u0, u, 2 * N * sizeof(double), hipMemcpyHostToDevice, g->stream));
+            if (g->prec == kFP64) launch_upload<double>(static_cast<double*>(g->f[g->cur]), g->g, df, g->stream);
+            else launch_upload<float>(static_cast<float*>(g->f[g->cur]), g->g, df, g->stream);
+            if (force_ibm) {
+                db = dalloc<double>(2 * N);
+                LIFE_HIP(hipMemcpyAsync(db, force_ibm, 2 * N * sizeof(double), hipMemcpyHostToDevice, g->stream));
+                launch_fibm_from_aos(g->fibm, g->g, db, g->stream);
+            } else {
+                launch_fill_f64(g->fibm, 0.0, 2 * g->g.S, g->stream);
+            }
+            LIFE_HIP(hipGetLastError());
+            LIFE_HIP(hipStreamSynchronize(g->stream));
+        } catch (...) {
+            dfree(df), dfree(db);
+            throw;
+        }
+        dfree(df), dfree(db);
+        g->explicit_state = true;
+        g->dense_clear_next = true;
+    });
+}
+
+int life_solve_lapack(int n, int bc, const double* A, const double* b, double* x) {
+    return guarded([&] {
+        if (n < 0 || bc < 0 || bc > n) throw std::runtime_error("bad system size");
+        std::vector<double> w;
+        std::vector<int> iw;
+        const int info = lu_solve_sparse(n, bc, A, b, x, w, iw);
+        if (info) throw std::runtime_error("singular matrix (zero pivot column " + std::to_string(info - 1) + ")");
+    });
+}
+
+int life_ibm_set_markers(life_grid* g, int n, const double* pos, const double* vel, const int* body, int nbodies,
+                         const int* body_flex) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        free_markers(g);
+        if (n < 0 || nbodies < 0) throw std::runtime_error("negative marker or body count");
+        std::vector<int> first(nbodies, -1), cnt(nbodies, 0);
+        for (int k = 0; k < n; k++) {
+            const int b = body[k];
+            if (b < 0 || b >= nbodies) throw std::runtime_error("marker body index out of range");
+            if (first[b] < 0) first[b] = k;
+            else if (first[b] + cnt[b] != k) throw std::runtime_error("markers of a body must be contiguous");
+            cnt[b]++;
+        }
+        for (int b = 0; b < nbodies; b++)
+            if (first[b] < 0) first[b] = 0;
+        g->body_first_h = first;
+        g->body_n_h = cnt;
+        g->body_flex_h.assign(body_flex, body_flex + nbodies);
+        g->nbodies = nbodies;
+        MarkerDev& m = g->md;
+        m.n = n;
+        m.pos = dalloc<double>(2 * (size_t)n);
+        m.vel = dalloc<double>(2 * (size_t)n);
+        m.force = dalloc<double>(2 * (size_t)n);
+        m.eps = dalloc<double>(n);
+        m.ds = dalloc<double>(n);
+        m.body = dalloc<int>(n);
+        m.supp_cnt = dalloc<int>(n);
+        m.supp_site = dalloc<int>((size_t)n * kSupp);
+        m.supp_ij = dalloc<int>((size_t)n * kSupp * 2);
+        m.supp_delta = dalloc<double>((size_t)n * kSupp);
+        g->body_first = dalloc<int>(nbodies);
+        g->body_n = dalloc<int>(nbodies);
+        g->body_flex = dalloc<int>(nbodies);
+        g->flags = dalloc<int>(2);
+        m.body_first = g->body_first;
+        m.body_n = g->body_n;
+        m.body_flex = g->body_flex;
+        m.overflow = g->flags;
+        SpreadScratch& sc = g->sc;
+        sc.cap = n * kSupp;
+        sc.site_count = dalloc<int>((size_t)g->g.N);
+        sc.site_uid = dalloc<int>((size_t)g->g.N);
+        sc.uniq = dalloc<int>(sc.cap);
+        sc.uniq_n = dalloc<int>(1);
+        sc.pair_slot = dalloc<int>(sc.cap);
+        sc.list = dalloc<int>((size_t)sc.cap * kMaxPerSite);
+        sc.overflow = g->flags + 1;
+        hipStream_t s = g->stream;
+        if (n) {
+            LIFE_HIP(hipMemcpyAsync(m.pos, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+            if (vel) LIFE_HIP(hipMemcpyAsync(m.vel, vel, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+            else launch_fill_f64(m.vel, 0.0, 2 * (long long)n, s);
+            LIFE_HIP(hipMemcpyAsync(m.body, body, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        }
+        if (nbodies) {
+            LIFE_HIP(hipMemcpyAsync(g->body_first, first.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, s));
+            LIFE_HIP(hipMemcpyAsync(g->body_n, cnt.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, s));
+            LIFE_HIP(hipMemcpyAsync(g->body_flex, body_flex, sizeof(int) * nbodies, hipMemcpyHostToDevice, s));
+        }
+        launch_fill_f64(m.force, 0.0, 2 * (long long)n, s);
+        launch_fill_f64(m.eps, 0.0, n, s);
+        launch_fill_f64(m.ds, 0.0, n, s);
+        launch_fill_i32(m.supp_cnt, 0, n, s);
+        launch_fill_i32(g->flags, 0, 2, s);
+        launch_fill_i32(sc.site_count, 0, g->g.N, s);
+        launch_fill_i32(sc.uniq_n, 0, 1, s);
+        LIFE_HIP(hipGetLastError());
+        LIFE_HIP(hipStreamSynchronize(s));
+        g->dense_clear_next = true;
+    });
+}
+
+int life_ibm_update_markers(life_grid* g, const double* pos, const double* vel) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        const size_t n = (size_t)g->md.n;
+        if (pos) LIFE_HIP(hipMemcpyAsync(g->md.pos, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice, g->stream));
+        if (vel) LIFE_HIP(hipMemcpyAsync(g->md.vel, vel, sizeof(double) * 2 * n, hipMemcpyHostToDevice, g->stream));
+    });
+}
+
+int life_ibm_find_support(life_grid* g, int flexible_only) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        ibm_find_support(g, flexible_only);
+        check_flags(g);
+    });
+}
+
+int life_ibm_epsilon_matrix(life_grid* g, int first, int n, double* A) {
+    return guarded([&] {
+        if (first < 0 || n < 0 || first + n > g->md.n) throw std::runtime_error("epsilon system out of range");
+        LIFE_HIP(hipSetDevice(g->device));
+        ibm_epsilon_matrix(g, first, n, A);
+    });
+}
+
+int life_ibm_set_epsilon(life_grid* g, const double* eps) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        LIFE_HIP(hipMemcpyAsync(g->md.eps, eps, sizeof(double) * g->md.n, hipMemcpyHostToDevice, g->stream));
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int life_ibm_interpolate(life_grid* g, double* force) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        ibm_interpolate(g);
+        if (force) {
+            LIFE_HIP(hipMemcpyAsync(force, g->md.force, sizeof(double) * 2 * g->md.n, hipMemcpyDeviceToHost,
+                                    g->stream));
+            LIFE_HIP(hipStreamSynchronize(g->stream));
+        }
+    });
+}
+
+int life_ibm_spread(life_grid* g) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        ibm_spread(g);
+    });
+}
+
+int life_ibm_download(life_grid* g, double* pos, double* vel, double* force, double* eps, double* ds, int* supp_count,
+                      int* supp_ij, double* supp_delta) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        const size_t n = (size_t)g->md.n;
+        hipStream_t s = g->stream;
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) LIFE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        };
+        cp(pos, g->md.pos, 16 * n);
+        cp(vel, g->md.vel, 16 * n);
+        cp(force, g->md.force, 16 * n);
+        cp(eps, g->md.eps, 8 * n);
+        cp(ds, g->md.ds, 8 * n);
+        cp(supp_count, g->md.supp_cnt, 4 * n);
+        cp(supp_ij, g->md.supp_ij, 4 * n * kSupp * 2);
+        cp(supp_delta, g->md.supp_delta, 8 * n * kSupp);
+        LIFE_HIP(hipStreamSynchronize(s));
+        check_flags(g);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Host runtime: the device lattice (life_grid) and shared helpers.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/life_hip.h"
+#include "launch.h"
+#include "life_dev.h"
+
+namespace life {
+
+void set_error(const std::string& msg);
+
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+
+#define LIFE_HIP(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) throw ::life::HipError{e_, std::string(#expr) + " @ " __FILE__}; \
+    } while (0)
+
+// Derived case scalars with the reference's rounding (Grid.cpp:1244-1260,
+// params.h:89-90).
+struct CaseScalars {
+    double omega, tau, nu, Dx, Dt, Dm, Drho, cs, cs2, cs4;
+};
+CaseScalars derive_scalars(const life_params& p);
+
+// GridClass::getRampCoefficient, Grid.cpp:548-556.
+double ramp_coefficient(const life_params& p, double Dt, int t);
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    LIFE_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+void dfree(void* p);
+
+}  // namespace life
+
+struct life_grid {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    life_params p{};
+    int prec = 0;
+    life::CaseScalars cs{};
+    life::GridScalars g{};
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;  // buffer holding the latest populations
+    double* rho0 = nullptr;
+    double* u0 = nullptr;
+    bool explicit_state = true;
+    double* fibm = nullptr;
+    bool dense_clear_next = false;
+    // boundaries
+    std::vector<life::BCSite> bc_host;
+    life::BCSite* bc = nullptr;
+    int nbc = 0;
+    bool convective = false;
+    std::vector<double> u_in_host, rho_in_host;
+    double* u_in = nullptr;
+    double* rho_in = nullptr;
+    double* delU = nullptr;
+    double* ucol = nullptr;
+    std::vector<int> type_host;
+    // markers
+    life::MarkerDev md{};
+    int nbodies = 0;
+    int* body_first = nullptr;
+    int* body_n = nullptr;
+    int* body_flex = nullptr;
+    std::vector<int> body_first_h, body_n_h, body_flex_h;
+    life::SpreadScratch sc{};
+    int* flags = nullptr;  // [0] support overflow, [1] spread overflow
+    double* epsA = nullptr;  // epsilon matrix scratch
+    size_t epsA_cap = 0;
+    // timing
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+
+    void* f_cur() { return f[cur]; }
+};
+
+namespace life {
+// Enqueue one lbmKernel on the grid stream; if ev_begin/ev_end are given,
+// they are recorded around the fused stream-collide launch.
+void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end);
+void ibm_interpolate(life_grid* g);
+void ibm_spread(life_grid* g);
+void ibm_find_support(life_grid* g, int flexible_only);
+void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host);
+void check_flags(life_grid* g);
+}  // namespace life

@@ -1,0 +1,124 @@
+"""GPU parity of the immersed-boundary coupling (ObjectsClass::objectKernel,
+Objects.cpp:26-321) for rigid and flexible bodies, plain method (FP64):
+supports, spacing, epsilon, marker forces, FEM state and lattice fields
+against the oracle at 1e-12 relative."""
+import numpy as np
+import pytest
+
+from life_amd import config as cfg
+from tests._util import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def small_cylinder():
+    return cfg.cylinder_channel(Nx=160, Ny=64, D=16.0, centre=(40.0, 31.5), u_max=0.06, Re=40.0, ramp_steps=0.0)
+
+
+def small_fsi(uni=1, cm=1):
+    return cfg.flexible_fsi(Nx=200, Ny=62, D=12.0, centre=(30.0, 30.0), L=42.0, h=2.4, n_el=6, u_mean=0.05,
+                            Re=60.0, ramp_steps=0.0, uni_epsilon=uni, central_moments=cm, stiffness=200.0)
+
+
+def small_canopy(cm=0):
+    p, g = cfg.canopy(Nx=320, Ny=96, n_filaments=3, L=40.0, n_el=10, min_spacing=30.0, x_range=(60, 220),
+                      ramp_steps=0.0, u_in=0.05, kb_range=(1e-2, 1e-1))
+    return p.replace(central_moments=cm, ux0_p=0.05), g
+
+
+def compare_markers(oc, gc):
+    mo = oc.markers()
+    mg = gc.markers()
+    cnt_o, ij_o, d_o = oc.supports()
+    assert np.array_equal(mg["supp_count"], cnt_o)
+    for k in range(len(cnt_o)):
+        n = cnt_o[k]
+        assert np.array_equal(mg["supp_ij"][k, :n], ij_o[k, :n])
+        assert np.array_equal(mg["supp_delta"][k, :n], d_o[k, :n])
+    assert rel_err(mg["ds"], mo["ds"]) <= TOL
+    assert rel_err(mg["eps"], mo["eps"]) <= TOL
+    assert rel_err(mg["pos"], mo["pos"]) <= TOL
+    assert rel_err(mg["force"], mo["force"]) <= TOL
+
+
+def compare_fields(oc, gc):
+    fo, ro, uo, bo = oc.grid()
+    fg, rg, ug, bg = gc.fields()
+    assert rel_err(bg, bo) <= TOL
+    assert rel_err(rg, ro) <= TOL
+    assert rel_err(ug, uo) <= TOL
+    assert rel_err(fg, fo) <= TOL
+
+
+def build(oracle, p, g, precision=0):
+    from life_amd.case import Case
+    oc = oracle.OracleCase(p, g)
+    gc = Case(p, g, precision=precision)
+    return oc, gc
+
+
+def test_initial_objects(oracle, gpu_lib):
+    """initialiseObjects: supports, ds and epsilon at t = 0 (Objects.cpp:941-961)."""
+    p, g = small_cylinder()
+    oc, gc = build(oracle, p, g)
+    assert gc.n_markers == oc.n_markers
+    compare_markers(oc, gc)
+
+
+def test_rigid_cylinder(oracle, gpu_lib):
+    p, g = small_cylinder()
+    oc, gc = build(oracle, p, g)
+    print("rigid cylinder markers", oc.n_markers, flush=True)
+    for _ in range(3):
+        oc.step(15)
+        gc.step(15)
+        compare_markers(oc, gc)
+        compare_fields(oc, gc)
+        assert rel_err(gc.total_force(), oc.total_force()) <= TOL
+
+
+@pytest.mark.parametrize("uni", [1, 0])
+def test_flexible_fsi(oracle, gpu_lib, uni):
+    p, g = small_fsi(uni=uni)
+    oc, gc = build(oracle, p, g)
+    print("fsi markers", oc.n_markers, "uni", uni, flush=True)
+    for _ in range(3):
+        oc.step(8)
+        gc.step(8)
+        co, cg = oc.coupling(), gc.coupling()
+        assert co["subIt"] == cg["subIt"]
+        assert abs(co["relax"] - cg["relax"]) <= TOL * max(1.0, abs(co["relax"]))
+        U_o = oc.fem(1)[0]
+        U_g = gc.fem(1)[0]
+        assert rel_err(U_g, U_o) <= TOL
+        compare_markers(oc, gc)
+        compare_fields(oc, gc)
+        assert rel_err(gc.total_force(), oc.total_force()) <= TOL
+
+
+def test_canopy_small(oracle, gpu_lib):
+    p, g = small_canopy()
+    oc, gc = build(oracle, p, g)
+    print("canopy markers", oc.n_markers, flush=True)
+    oc.step(10)
+    gc.step(10)
+    for b in range(oc.n_bodies):
+        assert rel_err(gc.fem(b)[0], oc.fem(b)[0]) <= TOL
+    compare_markers(oc, gc)
+    compare_fields(oc, gc)
+
+
+def test_layer1_epsilon_matrix(oracle, gpu_lib):
+    """life_ibm_epsilon_matrix + life_solve_lapack reproduce epsilon."""
+    from life_amd import lib
+    p, g = small_fsi(uni=1)
+    oc, gc = build(oracle, p, g)
+    n = gc.n_markers
+    gc.grid.n_markers = n
+    A = gc.grid.epsilon_matrix(0, n)
+    eps = lib.solve_lapack(A, np.ones(n))
+    assert rel_err(eps, oc.markers()["eps"]) <= TOL
+    # epsilon * A = 1 (Objects.cpp:303-306)
+    assert np.max(np.abs(A @ eps - 1.0)) < 1e-10
