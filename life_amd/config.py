@@ -135,16 +135,13 @@ def canopy(Nx=8192, Ny=4096, n_filaments=64, L=200.0, n_el=50, min_spacing=60.0,
     K_B = E I / (rho_f U^2 L^3), log-uniform; I = Dx h^3 / 12."""
     rng = random.Random(seed)
     lo, hi = x_range if x_range else (0.1 * Nx, 0.6 * Nx)
-    xs = []
-    tries = 0
-    while len(xs) < n_filaments:
-        tries += 1
-        if tries > 100000:
-            raise ValueError("cannot place %d filaments with spacing %g" % (n_filaments, min_spacing))
-        x = float(round(rng.uniform(lo, hi)))
-        if all(abs(x - y) >= min_spacing for y in xs):
-            xs.append(x)
-    xs.sort()
+    # uniform random positions with a minimum gap: sorted uniform samples on
+    # the range shortened by the gaps, then the k-th shifted by k gaps
+    free = (hi - lo) - (n_filaments - 1) * min_spacing
+    if n_filaments and free < 0:
+        raise ValueError("cannot place %d filaments with spacing %g" % (n_filaments, min_spacing))
+    base = sorted(rng.uniform(0.0, free) for _ in range(n_filaments))
+    xs = [float(math.floor(lo + b + k * min_spacing)) for k, b in enumerate(base)]
     I = 1.0 * h ** 3 / 12.0
     lines = []
     for k, x in enumerate(xs):

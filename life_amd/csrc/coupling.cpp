@@ -43,6 +43,7 @@ struct life_case {
     CaseScalars cs{};
     int t = 0;
     std::vector<Body> bodies;
+    std::vector<Body*> flex_bodies;  // flexible bodies in body order
     int nm = 0;
     std::vector<double> pos, vel, force, eps, ds;
     std::vector<int> mbody;
@@ -201,8 +202,10 @@ void finalise_markers(life_case* c) {
     for (auto& b : c->bodies) b.mpos.clear();
 }
 
-// ObjectsClass::computeEpsilon (Objects.cpp:235-321): device matrix, host
-// canonical solve of A eps = 1.
+// ObjectsClass::computeEpsilon (Objects.cpp:235-321).  UNI_EPSILON and the
+// initial call: dense device matrix, host canonical solve of A eps = 1.
+// Later calls (flexible bodies only): banded systems assembled and solved
+// on the device, with the dense host path for any system that is not banded.
 void compute_epsilon(life_case* c, bool initial) {
     auto solve = [&](int first, int n) {
         c->A.resize((size_t)n * n);
@@ -211,30 +214,44 @@ void compute_epsilon(life_case* c, bool initial) {
         c->sol.assign(n, 0.0);
         lu_solve_sparse(n, 0, c->A.data(), c->rhs.data(), c->sol.data(), c->luw, c->luiw);
         std::memcpy(&c->eps[first], c->sol.data(), sizeof(double) * n);
+        LIFE_HIP(hipMemcpyAsync(c->g->md.eps + first, &c->eps[first], sizeof(double) * n, hipMemcpyHostToDevice,
+                                c->g->stream));
     };
     if (c->p.uni_epsilon) {
         solve(0, c->nm);
-    } else {
+    } else if (initial) {
         for (auto& b : c->bodies)
-            if ((initial || b.flex == 0) && b.n > 0) solve(b.first, b.n);
+            if (b.n > 0) solve(b.first, b.n);
+    } else {
+        for (int k : ibm_epsilon_band(c->g)) solve(c->g->eps_first_h[k], c->g->eps_n_h[k]);
     }
-    LIFE_HIP(hipMemcpyAsync(c->g->md.eps, c->eps.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice, c->g->stream));
 }
 
+// Marker forces (and epsilon, which the FEM load vector uses) to the host.
 void download_forces(life_case* c) {
     LIFE_HIP(hipMemcpyAsync(c->force.data(), c->g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
                             c->g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->eps.data(), c->g->md.eps, sizeof(double) * c->nm, hipMemcpyDeviceToHost, c->g->stream));
     LIFE_HIP(hipStreamSynchronize(c->g->stream));
+}
+
+// Run fn(body) for every flexible body.  Bodies are independent (disjoint
+// FEM state and marker ranges), so they run on host threads; anything
+// order-dependent is reduced afterwards in body order.
+template <typename Fn>
+void for_flexible(life_case* c, Fn&& fn) {
+    const int nf = (int)c->flex_bodies.size();
+#pragma omp parallel for schedule(dynamic, 1) if (nf > 1)
+    for (int q = 0; q < nf; q++) fn(*c->flex_bodies[q]);
 }
 
 // ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
 void recompute_object_vals(life_case* c) {
     if (c->subIt == 0) {
-        for (auto& b : c->bodies)
-            if (b.flex == 0) {
-                b.fem->reset_values();
-                b.fem->predictor(c->t, c->view(b));
-            }
+        for_flexible(c, [&](Body& b) {
+            b.fem->reset_values();
+            b.fem->predictor(c->t, c->view(b));
+        });
     } else {
         if (c->subIt == 1) {
             const double a = std::fabs(c->relax), r = a < c->p.relaxMax ? a : c->p.relaxMax;
@@ -242,8 +259,8 @@ void recompute_object_vals(life_case* c) {
         } else {
             c->relax = -c->relax * c->subNum / c->subDen;
         }
-        for (auto& b : c->bodies)
-            if (b.flex == 0) b.fem->relax(c->relax, c->view(b));
+        const double w = c->relax;
+        for_flexible(c, [&](Body& b) { b.fem->relax(w, c->view(b)); });
     }
     life_grid* g = c->g;
     LIFE_HIP(hipMemcpyAsync(g->md.pos, c->pos.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
@@ -263,14 +280,13 @@ void object_kernel(life_case* c) {
         if (!c->hasFlex) break;
         download_forces(c);
         // femKernel, Objects.cpp:63-99 (ORDERED: sums in body order)
+        for_flexible(c, [&](Body& b) { b.fem->dynamic(c->view(b)); });
         double res = 0.0, num = 0.0, den = 0.0;
-        for (auto& b : c->bodies)
-            if (b.flex == 0) {
-                b.fem->dynamic(c->view(b));
-                res += b.fem->sub_res();
-                num += b.fem->sub_num();
-                den += b.fem->sub_den();
-            }
+        for (Body* b : c->flex_bodies) {
+            res += b->fem->sub_res();
+            num += b->fem->sub_num();
+            den += b->fem->sub_den();
+        }
         c->subRes = std::sqrt(res) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
         c->subNum = num;
         c->subDen = den;
@@ -323,6 +339,7 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
                 c->hasFlex = true;
                 b.fem.reset(new FemBody(fs, b.fpos, b.fgeom, b.fangle, b.fnel, b.fbc, b.frho, b.fE, b.n));
                 c->simDOFs += b.fem->ndof();
+                c->flex_bodies.push_back(&b);
             }
         }
         if (life_grid_create(p, precision, device, &c->g)) throw std::runtime_error(life_last_error());
@@ -335,6 +352,15 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
             // ObjectsClass::initialiseObjects, Objects.cpp:941-961
             ibm_find_support(c->g, 0);
             compute_epsilon(c, true);
+            if (c->hasFlex && !p->uni_epsilon) {
+                std::vector<int> first, n;
+                for (auto& b : c->bodies)
+                    if (b.flex == 0 && b.n > 0) {
+                        first.push_back(b.first);
+                        n.push_back(b.n);
+                    }
+                ibm_epsilon_band_setup(c->g, first, n, 8);
+            }
             LIFE_HIP(hipMemcpyAsync(c->ds.data(), c->g->md.ds, sizeof(double) * c->nm, hipMemcpyDeviceToHost,
                                     c->g->stream));
             LIFE_HIP(hipStreamSynchronize(c->g->stream));

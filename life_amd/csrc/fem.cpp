@@ -23,19 +23,27 @@ double dot(const double* a, const double* b, int n) {
     return r;
 }
 
-// Transpose(T) * A * T (matrix products accumulate k = 0..5 from 0.0).
+// The element transformation T (FEMElement.cpp:255-262) is block diagonal:
+// rows/columns {0,1} and {3,4} hold the rotation, 2 and 5 the identity.
+// The reference multiplies the full 6x6 matrices, accumulating k = 0..5
+// from 0.0; the terms with a structural zero of T add an exact zero, so
+// summing only the structurally non-zero terms in increasing k gives the
+// same values.  kNZ[i] lists the structurally non-zero k of row/column i.
+constexpr int kNZ[6][2] = {{0, 1}, {0, 1}, {2, -1}, {3, 4}, {3, 4}, {5, -1}};
+
+// Transpose(T) * A * T
 void rotate6(const double T[6][6], const double A[6][6], double out[6][6]) {
     double tmp[6][6];
     for (int i = 0; i < 6; i++)
         for (int j = 0; j < 6; j++) {
             double s = 0.0;
-            for (int k = 0; k < 6; k++) s += T[k][i] * A[k][j];
+            for (int q = 0; q < 2 && kNZ[i][q] >= 0; q++) s += T[kNZ[i][q]][i] * A[kNZ[i][q]][j];
             tmp[i][j] = s;
         }
     for (int i = 0; i < 6; i++)
         for (int j = 0; j < 6; j++) {
             double s = 0.0;
-            for (int k = 0; k < 6; k++) s += tmp[i][k] * T[k][j];
+            for (int q = 0; q < 2 && kNZ[j][q] >= 0; q++) s += tmp[i][kNZ[j][q]] * T[kNZ[j][q]][j];
             out[i][j] = s;
         }
 }
@@ -43,7 +51,7 @@ void rotate6(const double T[6][6], const double A[6][6], double out[6][6]) {
 void mul_t6(const double T[6][6], const double* v, double* out) {  // Transpose(T) * v
     for (int i = 0; i < 6; i++) {
         double s = 0.0;
-        for (int j = 0; j < 6; j++) s += T[j][i] * v[j];
+        for (int q = 0; q < 2 && kNZ[i][q] >= 0; q++) s += T[kNZ[i][q]][i] * v[kNZ[i][q]];
         out[i] = s;
     }
 }
@@ -51,7 +59,7 @@ void mul_t6(const double T[6][6], const double* v, double* out) {  // Transpose(
 void mul6(const double T[6][6], const double* v, double* out) {  // T * v
     for (int i = 0; i < 6; i++) {
         double s = 0.0;
-        for (int j = 0; j < 6; j++) s += T[i][j] * v[j];
+        for (int q = 0; q < 2 && kNZ[i][q] >= 0; q++) s += T[i][kNZ[i][q]] * v[kNZ[i][q]];
         out[i] = s;
     }
 }

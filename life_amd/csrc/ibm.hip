@@ -87,6 +87,134 @@ __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, d
     A[t] = a * m.ds[mj];
 }
 
+// ---------------------------------------------------------------- epsilon, banded
+// Batched form of computeEpsilon for systems whose matrix is banded (every
+// filament: A_ij = 0 unless markers i and j are within 3 lattice spacings).
+// One workgroup per system assembles rows i, columns |i-j| <= KB into band
+// storage (row width 3*KB+1: room for the fill-in of partial pivoting) and
+// flags the system if any pair outside the band is close enough to couple;
+// flagged systems are solved on the host with the dense path instead.
+__global__ void k_eps_band_assemble(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                    const int* __restrict__ sys_n, int KB, double* __restrict__ band,
+                                    const long long* __restrict__ band_off, int* __restrict__ outside) {
+    const int sys = blockIdx.x;
+    const int first = sys_first[sys], n = sys_n[sys];
+    const int W = 3 * KB + 1;
+    double* B = band + band_off[sys];
+    const double Dx = g.Dx;
+    // band entries (and zero fill-in columns)
+    for (long long t = threadIdx.x; t < (long long)n * W; t += blockDim.x) {
+        const int r = (int)(t / W), q = (int)(t - (long long)r * W);
+        const int c = r - KB + q;
+        double a = 0.0;
+        if (q <= 2 * KB && c >= 0 && c < n) {
+            const int mi = first + r, mj = first + c;
+            const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
+            const int cnt = m.supp_cnt[mi];
+            for (int s = 0; s < cnt; s++) {
+                const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
+                const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
+                a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
+            }
+            a = a * m.ds[mj];
+        }
+        B[t] = a;
+    }
+    // pairs outside the band that could couple (support of i within 1.5 of
+    // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings)
+    int far = 0;
+    for (long long t = threadIdx.x; t < (long long)n * n; t += blockDim.x) {
+        const int r = (int)(t / n), c = (int)(t - (long long)r * n);
+        if (c - r <= KB && r - c <= KB) continue;
+        const int mi = first + r, mj = first + c;
+        if (fabs(m.pos[2 * mi] - m.pos[2 * mj]) < 3.01 * Dx && fabs(m.pos[2 * mi + 1] - m.pos[2 * mj + 1]) < 3.01 * Dx)
+            far = 1;
+    }
+    if (far) outside[sys] = 1;
+}
+
+// Canonical elimination (partial pivoting, first maximum) of one banded
+// system per workgroup, right-hand side of ones, writing epsilon.  The
+// pivot search and the back substitution run on one lane in the canonical
+// order; the independent row updates of each step are spread over lanes,
+// each still computing a_ij - l_i a_kj exactly once, so every value equals
+// the dense elimination's.
+__global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
+                                                       const int* __restrict__ outside, int KB,
+                                                       double* __restrict__ band,
+                                                       const long long* __restrict__ band_off,
+                                                       double* __restrict__ ywork, double* __restrict__ eps) {
+    const int sys = blockIdx.x;
+    if (outside[sys]) return;
+    const int first = sys_first[sys], n = sys_n[sys];
+    const int kl = KB, ku = KB, W = 3 * KB + 1;
+    double* B = band + band_off[sys];
+    double* y = ywork + first;
+    auto at = [&](int r, int c) -> double& { return B[(long long)r * W + (c - r + kl)]; };
+    for (int r = threadIdx.x; r < n; r += blockDim.x) y[r] = 1.0;
+    __shared__ int piv_row;
+    __shared__ int skip;
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        const int ilast = min(k + kl, n - 1);
+        const int jlast = min(k + kl + ku, n - 1);
+        if (threadIdx.x == 0) {
+            int p = k;
+            double best = fabs(at(k, k));
+            for (int i = k + 1; i <= ilast; i++) {
+                const double a = fabs(at(i, k));
+                if (a > best) {
+                    best = a;
+                    p = i;
+                }
+            }
+            piv_row = p;
+            skip = (best == 0.0);
+        }
+        __syncthreads();
+        if (skip) {
+            __syncthreads();
+            continue;
+        }
+        const int p = piv_row;
+        if (p != k) {
+            for (int j = k + threadIdx.x; j <= jlast; j += blockDim.x) {
+                const double t = at(k, j);
+                at(k, j) = at(p, j);
+                at(p, j) = t;
+            }
+            if (threadIdx.x == 0) {
+                const double t = y[k];
+                y[k] = y[p];
+                y[p] = t;
+            }
+        }
+        __syncthreads();
+        const double pv = at(k, k);
+        const int nr = ilast - k, nc = jlast - k;
+        // multipliers need a_ik before row i is updated: each lane owns whole rows
+        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+            const int i = k + 1 + r;
+            const double l = at(i, k) / pv;
+            at(i, k) = l;
+            for (int c = 0; c < nc; c++) {
+                const int j = k + 1 + c;
+                at(i, j) = at(i, j) - l * at(k, j);
+            }
+            y[i] = y[i] - l * y[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[i];
+            const int jlast = min(i + kl + ku, n - 1);
+            for (int j = i + 1; j <= jlast; j++) s = s - at(i, j) * eps[first + j];
+            eps[first + i] = s / at(i, i);
+        }
+    }
+}
+
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
 // the supports (GridClass::macroscopic values: force_xy only) interpolated
 // with the support deltas, then F = 2 (Dt/Dx rho_I U - (rho u)_I).
@@ -186,6 +314,13 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
     fibm[site] = fx;
     fibm[S + site] = fy;
     sc.site_count[site] = 0;
+}
+
+void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n, int nsys,
+                     int KB, double* band, const long long* band_off, int* outside, double* ywork, hipStream_t s) {
+    if (nsys <= 0) return;
+    k_eps_band_assemble<<<nsys, 256, 0, s>>>(m, g, sys_first, sys_n, KB, band, band_off, outside);
+    k_eps_band_solve<<<nsys, 64, 0, s>>>(sys_first, sys_n, outside, KB, band, band_off, ywork, m.eps);
 }
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {

@@ -170,6 +170,51 @@ void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host) {
     LIFE_HIP(hipStreamSynchronize(g->stream));
 }
 
+void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const std::vector<int>& n, int KB) {
+    for (void* p : {(void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
+                    (void*)g->eps_band, (void*)g->eps_y})
+        dfree(p);
+    g->eps_nsys = (int)first.size();
+    g->eps_KB = KB;
+    g->eps_first_h = first;
+    g->eps_n_h = n;
+    std::vector<long long> off(first.size());
+    long long tot = 0;
+    for (size_t k = 0; k < first.size(); k++) {
+        off[k] = tot;
+        tot += (long long)n[k] * (3 * KB + 1);
+    }
+    g->eps_first = dalloc<int>(first.size());
+    g->eps_n = dalloc<int>(first.size());
+    g->eps_off = dalloc<long long>(first.size());
+    g->eps_outside = dalloc<int>(first.size());
+    g->eps_band = dalloc<double>((size_t)tot);
+    g->eps_y = dalloc<double>((size_t)(g->md.n > 0 ? g->md.n : 1));
+    if (!first.empty()) {
+        LIFE_HIP(hipMemcpyAsync(g->eps_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice,
+                                g->stream));
+        LIFE_HIP(hipMemcpyAsync(g->eps_n, n.data(), sizeof(int) * n.size(), hipMemcpyHostToDevice, g->stream));
+        LIFE_HIP(hipMemcpyAsync(g->eps_off, off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice,
+                                g->stream));
+    }
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+}
+
+std::vector<int> ibm_epsilon_band(life_grid* g) {
+    std::vector<int> dense;
+    if (g->eps_nsys <= 0) return dense;
+    launch_fill_i32(g->eps_outside, 0, g->eps_nsys, g->stream);
+    launch_eps_band(g->md, g->g, g->eps_first, g->eps_n, g->eps_nsys, g->eps_KB, g->eps_band, g->eps_off,
+                    g->eps_outside, g->eps_y, g->stream);
+    LIFE_HIP(hipGetLastError());
+    std::vector<int> out(g->eps_nsys);
+    LIFE_HIP(hipMemcpyAsync(out.data(), g->eps_outside, sizeof(int) * out.size(), hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    for (int k = 0; k < g->eps_nsys; k++)
+        if (out[k]) dense.push_back(k);
+    return dense;
+}
+
 void check_flags(life_grid* g) {
     if (!g->flags) return;
     int h[2] = {0, 0};
@@ -349,7 +394,9 @@ int life_grid_destroy(life_grid* g) {
         if (g->stream) (void)hipStreamSynchronize(g->stream);
         free_markers(g);
         for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm, (void*)g->bc,
-                        (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA})
+                        (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA,
+                        (void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
+                        (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
         for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1})
             if (e) (void)hipEventDestroy(e);

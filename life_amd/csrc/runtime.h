@@ -82,6 +82,15 @@ struct life_grid {
     int* flags = nullptr;  // [0] support overflow, [1] spread overflow
     double* epsA = nullptr;  // epsilon matrix scratch
     size_t epsA_cap = 0;
+    // banded epsilon systems (flexible bodies)
+    int eps_nsys = 0, eps_KB = 0;
+    int* eps_first = nullptr;
+    int* eps_n = nullptr;
+    long long* eps_off = nullptr;
+    int* eps_outside = nullptr;
+    double* eps_band = nullptr;
+    double* eps_y = nullptr;
+    std::vector<int> eps_first_h, eps_n_h;
     // timing
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
 
@@ -96,5 +105,10 @@ void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);
 void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host);
+// Declare the banded epsilon systems [first[k], first[k]+n[k]) once.
+void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const std::vector<int>& n, int KB);
+// Assemble and solve them on the device (epsilon lands in md.eps); returns
+// the indices of systems that were not banded and still need a solve.
+std::vector<int> ibm_epsilon_band(life_grid* g);
 void check_flags(life_grid* g);
 }  // namespace life
