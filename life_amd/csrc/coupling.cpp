@@ -7,7 +7,10 @@
 // on the host; everything on the lattice runs on the device.
 //
 // Compiled with -ffp-contract=off.
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -35,6 +38,33 @@ struct Body {
     std::string fnel, fbc;
 };
 
+// Optional phase timing (environment LIFE_TIMING=1): synchronises the
+// stream at each phase boundary, so it is for diagnosis only.
+struct PhaseTimes {
+    bool on = std::getenv("LIFE_TIMING") != nullptr;
+    double t[6] = {0, 0, 0, 0, 0, 0};  // lbm, recompute, interp+d2h, fem, spread, steps
+    long subits = 0;
+    std::chrono::steady_clock::time_point last;
+    void start(life_grid* g) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        last = std::chrono::steady_clock::now();
+    }
+    void lap(life_grid* g, int k) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        const auto now = std::chrono::steady_clock::now();
+        t[k] += std::chrono::duration<double, std::milli>(now - last).count();
+        last = now;
+    }
+    void report() const {
+        if (!on || t[5] == 0) return;
+        std::fprintf(stderr, "life timing over %.0f steps (%ld subiterations), ms/step: lbm %.3f recompute %.3f "
+                     "interp %.3f fem %.3f spread %.3f\n", t[5], subits, t[0] / t[5], t[1] / t[5], t[2] / t[5],
+                     t[3] / t[5], t[4] / t[5]);
+    }
+};
+
 }  // namespace
 
 struct life_case {
@@ -55,6 +85,8 @@ struct life_case {
     std::vector<int> luiw;
     std::vector<double> rhs, sol;
     bool forces_on_host = false;
+    std::unique_ptr<PhaseTimes> timing{new PhaseTimes()};
+    std::vector<hipEvent_t> kev;  // per-step stream-collide events of life_case_step_timed
 
     MarkerView view(const Body& b) {
         return MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};
@@ -274,13 +306,18 @@ void object_kernel(life_case* c) {
     const int MAXIT = 20;
     c->subIt = 0;
     c->forces_on_host = false;
+    PhaseTimes& T = *c->timing;
     do {
         if (c->hasFlex) recompute_object_vals(c);
+        T.lap(c->g, 1);
         ibm_interpolate(c->g);
         if (!c->hasFlex) break;
         download_forces(c);
+        T.lap(c->g, 2);
+        T.subits++;
         // femKernel, Objects.cpp:63-99 (ORDERED: sums in body order)
         for_flexible(c, [&](Body& b) { b.fem->dynamic(c->view(b)); });
+        T.lap(c->g, 3);
         double res = 0.0, num = 0.0, den = 0.0;
         for (Body* b : c->flex_bodies) {
             res += b->fem->sub_res();
@@ -293,6 +330,7 @@ void object_kernel(life_case* c) {
         c->subIt++;
     } while (c->subIt < MAXIT && c->subRes > c->p.subTol);
     ibm_spread(c->g);
+    T.lap(c->g, 4);
     if (c->hasFlex) {
         // the last FEM solve moved the markers (FEMBody.cpp:63-64); the
         // spread above used the supports found before it, as the reference
@@ -306,16 +344,14 @@ void object_kernel(life_case* c) {
     if (c->subIt == MAXIT) throw std::runtime_error("Subiteration scheme hit 20 iterations");
 }
 
-void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, double* kernel_ms) {
+void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    PhaseTimes& T = *c->timing;
+    T.start(c->g);
     c->t++;
     lbm_step(c->g, c->t, ev_begin, ev_end);
+    T.lap(c->g, 0);
     if (c->hasIBM) object_kernel(c);
-    if (kernel_ms && ev_begin) {
-        LIFE_HIP(hipEventSynchronize(ev_end));
-        float ms = 0.0f;
-        LIFE_HIP(hipEventElapsedTime(&ms, ev_begin, ev_end));
-        *kernel_ms += ms;
-    }
+    T.t[5] += 1;
 }
 
 }  // namespace
@@ -378,7 +414,12 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
 int life_case_destroy(life_case* c) {
     if (!c) return 0;
     int rc = 0;
-    if (c->g) rc = life_grid_destroy(c->g);
+    c->timing->report();
+    if (c->g) {
+        (void)hipSetDevice(c->g->device);
+        for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+        rc = life_grid_destroy(c->g);
+    }
     delete c;
     return rc;
 }
@@ -386,7 +427,7 @@ int life_case_destroy(life_case* c) {
 int life_case_step(life_case* c, int nsteps) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(c->g->device));
-        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr, nullptr);
+        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr);
         LIFE_HIP(hipStreamSynchronize(c->g->stream));
         if (c->hasIBM) check_flags(c->g);
     });
@@ -397,13 +438,25 @@ int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_
         life_grid* g = c->g;
         LIFE_HIP(hipSetDevice(g->device));
         LIFE_HIP(hipStreamSynchronize(g->stream));
-        double kms = 0.0;
+        // one event pair per step around the stream-collide launch, read
+        // after the last step so the host never waits inside the loop
+        while ((int)c->kev.size() < 2 * nsteps) {
+            hipEvent_t e;
+            LIFE_HIP(hipEventCreate(&e));
+            c->kev.push_back(e);
+        }
         LIFE_HIP(hipEventRecord(g->ev_a, g->stream));
-        for (int s = 0; s < nsteps; s++) case_step(c, g->ev_k0, g->ev_k1, &kms);
+        for (int s = 0; s < nsteps; s++) case_step(c, c->kev[2 * s], c->kev[2 * s + 1]);
         LIFE_HIP(hipEventRecord(g->ev_b, g->stream));
         LIFE_HIP(hipEventSynchronize(g->ev_b));
         float ms = 0.0f;
         LIFE_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
+        double kms = 0.0;
+        for (int s = 0; s < nsteps; s++) {
+            float k = 0.0f;
+            LIFE_HIP(hipEventElapsedTime(&k, c->kev[2 * s], c->kev[2 * s + 1]));
+            kms += k;
+        }
         if (ms_total) *ms_total = ms;
         if (ms_kernel) *ms_kernel = kms;
         if (c->hasIBM) check_flags(g);

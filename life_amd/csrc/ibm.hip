@@ -49,22 +49,30 @@ __global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
 }
 
 // IBMNodeClass::computeDs (IBMNode.cpp:182-204): distance (lattice units)
-// to the nearest other marker of the same body, starting from 10.
+// to the nearest other marker of the same body, starting from 10.  One
+// workgroup per body; the body's positions are staged in LDS when they fit.
+constexpr int kDsLdsMarkers = 2048;
 __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m.n) return;
-    const int b = m.body[k];
+    extern __shared__ double spos[];
+    const int b = blockIdx.x;
     if (flexible_only && m.body_flex[b] != 0) return;
-    const double x = m.pos[2 * k], y = m.pos[2 * k + 1];
-    double ds = 10.0;
     const int first = m.body_first[b], cnt = m.body_n[b];
-    for (int q = first; q < first + cnt; q++) {
-        if (q == k) continue;
-        const double dx = x - m.pos[2 * q], dy = y - m.pos[2 * q + 1];
-        const double mag = sqrt(0.0 + dx * dx + dy * dy) / g.Dx;
-        if (mag < ds) ds = mag;
+    const bool lds = cnt <= kDsLdsMarkers;
+    if (lds)
+        for (int t = threadIdx.x; t < 2 * cnt; t += blockDim.x) spos[t] = m.pos[2 * first + t];
+    __syncthreads();
+    const double* P = lds ? spos : m.pos + 2 * first;
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const double x = P[2 * k], y = P[2 * k + 1];
+        double ds = 10.0;
+        for (int q = 0; q < cnt; q++) {
+            if (q == k) continue;
+            const double dx = x - P[2 * q], dy = y - P[2 * q + 1];
+            const double mag = sqrt(0.0 + dx * dx + dy * dy) / g.Dx;
+            if (mag < ds) ds = mag;
+        }
+        m.ds[first + k] = ds;
     }
-    m.ds[k] = ds;
 }
 
 // A_ij of ObjectsClass::computeEpsilon (Objects.cpp:271-300) for the system
@@ -88,6 +96,7 @@ __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, d
 }
 
 // ---------------------------------------------------------------- epsilon, banded
+constexpr int kEpsLdsMarkers = 2048;  // systems up to this size stage positions in LDS
 // Batched form of computeEpsilon for systems whose matrix is banded (every
 // filament: A_ij = 0 unless markers i and j are within 3 lattice spacings).
 // One workgroup per system assembles rows i, columns |i-j| <= KB into band
@@ -121,96 +130,127 @@ __global__ void k_eps_band_assemble(MarkerDev m, GridScalars g, const int* __res
         B[t] = a;
     }
     // pairs outside the band that could couple (support of i within 1.5 of
-    // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings)
+    // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings);
+    // the test is symmetric, so only c > r + KB is visited
+    extern __shared__ double spos[];
+    const bool lds = n <= kEpsLdsMarkers;
+    if (lds)
+        for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) spos[t] = m.pos[2 * first + t];
+    __syncthreads();
+    const double* P = lds ? spos : m.pos + 2 * first;
+    const double lim = 3.01 * Dx;
     int far = 0;
-    for (long long t = threadIdx.x; t < (long long)n * n; t += blockDim.x) {
-        const int r = (int)(t / n), c = (int)(t - (long long)r * n);
-        if (c - r <= KB && r - c <= KB) continue;
-        const int mi = first + r, mj = first + c;
-        if (fabs(m.pos[2 * mi] - m.pos[2 * mj]) < 3.01 * Dx && fabs(m.pos[2 * mi + 1] - m.pos[2 * mj + 1]) < 3.01 * Dx)
-            far = 1;
+    for (int r = threadIdx.x / 64; r < n && !far; r += blockDim.x / 64) {
+        const double xr = P[2 * r], yr = P[2 * r + 1];
+        for (int c = r + KB + 1 + (threadIdx.x & 63); c < n; c += 64)
+            if (fabs(xr - P[2 * c]) < lim && fabs(yr - P[2 * c + 1]) < lim) far = 1;
     }
     if (far) outside[sys] = 1;
 }
 
 // Canonical elimination (partial pivoting, first maximum) of one banded
-// system per workgroup, right-hand side of ones, writing epsilon.  The
-// pivot search and the back substitution run on one lane in the canonical
-// order; the independent row updates of each step are spread over lanes,
-// each still computing a_ij - l_i a_kj exactly once, so every value equals
-// the dense elimination's.
+// system per wavefront, right-hand side of ones, writing epsilon.
+//
+// With lower and upper bandwidth KB, every non-zero that step k of the
+// dense elimination reads or writes lies in rows k..k+KB and columns
+// k..k+2KB (row swaps widen the upper band by KB).  Lane c of the wavefront
+// keeps column k+c of that window in registers (w[r] = a(k+r, k+c)); each
+// step broadcasts column k, picks the pivot, swaps, forms the multipliers,
+// updates the other columns (each a_ij - l_i a_kj exactly once from the same
+// operands as the dense algorithm), emits row k of U, and slides the window
+// one row and one column.  Back substitution then runs on one lane in
+// increasing column order.  The skipped dense operations all involve an
+// exact zero, so the values equal the dense elimination's.
+template <int KB>
 __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
-                                                       const int* __restrict__ outside, int KB,
-                                                       double* __restrict__ band,
+                                                       const int* __restrict__ outside, double* __restrict__ band,
                                                        const long long* __restrict__ band_off,
                                                        double* __restrict__ ywork, double* __restrict__ eps) {
+    constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
     const int sys = blockIdx.x;
     if (outside[sys]) return;
     const int first = sys_first[sys], n = sys_n[sys];
-    const int kl = KB, ku = KB, W = 3 * KB + 1;
     double* B = band + band_off[sys];
     double* y = ywork + first;
-    auto at = [&](int r, int c) -> double& { return B[(long long)r * W + (c - r + kl)]; };
-    for (int r = threadIdx.x; r < n; r += blockDim.x) y[r] = 1.0;
-    __shared__ int piv_row;
-    __shared__ int skip;
-    __syncthreads();
+    const int c = threadIdx.x;
+    auto A0 = [&](int r, int col) -> double {
+        if (c >= C || r >= n || col < 0 || col >= n) return 0.0;
+        const int q = col - r + KB;
+        return (q >= 0 && q < W) ? B[(long long)r * W + q] : 0.0;
+    };
+    double w[R], yw[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        w[r] = A0(r, c);
+        yw[r] = r < n ? 1.0 : 0.0;
+    }
     for (int k = 0; k < n; k++) {
-        const int ilast = min(k + kl, n - 1);
-        const int jlast = min(k + kl + ku, n - 1);
-        if (threadIdx.x == 0) {
-            int p = k;
-            double best = fabs(at(k, k));
-            for (int i = k + 1; i <= ilast; i++) {
-                const double a = fabs(at(i, k));
+        const int rmax = min(KB, n - 1 - k);
+        double col[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) col[r] = __shfl(w[r], 0);
+        int p = 0;
+        double best = fabs(col[0]);
+#pragma unroll
+        for (int r = 1; r < R; r++)
+            if (r <= rmax) {
+                const double a = fabs(col[r]);
                 if (a > best) {
                     best = a;
-                    p = i;
+                    p = r;
                 }
             }
-            piv_row = p;
-            skip = (best == 0.0);
-        }
-        __syncthreads();
-        if (skip) {
-            __syncthreads();
-            continue;
-        }
-        const int p = piv_row;
-        if (p != k) {
-            for (int j = k + threadIdx.x; j <= jlast; j += blockDim.x) {
-                const double t = at(k, j);
-                at(k, j) = at(p, j);
-                at(p, j) = t;
+        if (best != 0.0) {
+            if (p != 0) {
+                double tw = w[0], tc = col[0], ty = yw[0];
+#pragma unroll
+                for (int r = 1; r < R; r++)
+                    if (r == p) {
+                        w[0] = w[r];
+                        w[r] = tw;
+                        col[0] = col[r];
+                        col[r] = tc;
+                        yw[0] = yw[r];
+                        yw[r] = ty;
+                    }
             }
-            if (threadIdx.x == 0) {
-                const double t = y[k];
-                y[k] = y[p];
-                y[p] = t;
-            }
+            const double piv = col[0];
+#pragma unroll
+            for (int r = 1; r < R; r++)
+                if (r <= rmax) {
+                    const double l = col[r] / piv;
+                    if (c == 0) w[r] = l;
+                    else w[r] = w[r] - l * w[0];
+                    yw[r] = yw[r] - l * yw[0];
+                }
         }
-        __syncthreads();
-        const double pv = at(k, k);
-        const int nr = ilast - k, nc = jlast - k;
-        // multipliers need a_ik before row i is updated: each lane owns whole rows
-        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-            const int i = k + 1 + r;
-            const double l = at(i, k) / pv;
-            at(i, k) = l;
-            for (int c = 0; c < nc; c++) {
-                const int j = k + 1 + c;
-                at(i, j) = at(i, j) - l * at(k, j);
-            }
-            y[i] = y[i] - l * y[k];
+        // row k of U (in place of row k of the band) and the final y_k
+        if (c < C && k + c < n) B[(long long)k * W + (c + KB)] = w[0];
+        if (c == 0) y[k] = yw[0];
+        // slide: drop row k and column k, take in row k+R
+#pragma unroll
+        for (int r = 0; r < R - 1; r++) {
+            w[r] = __shfl_down(w[r + 1], 1);
+            yw[r] = yw[r + 1];
         }
-        __syncthreads();
+        w[R - 1] = A0(k + R, k + 1 + c);
+        yw[R - 1] = k + R < n ? 1.0 : 0.0;
     }
-    if (threadIdx.x == 0) {
+    if (c == 0) {
+        double xw[2 * KB];
+#pragma unroll
+        for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
         for (int i = n - 1; i >= 0; i--) {
+            const double* Ui = B + (long long)i * W + KB;
             double s = y[i];
-            const int jlast = min(i + kl + ku, n - 1);
-            for (int j = i + 1; j <= jlast; j++) s = s - at(i, j) * eps[first + j];
-            eps[first + i] = s / at(i, i);
+#pragma unroll
+            for (int q = 0; q < 2 * KB; q++)
+                if (i + 1 + q < n) s = s - Ui[1 + q] * xw[q];
+            const double x = s / Ui[0];
+            eps[first + i] = x;
+#pragma unroll
+            for (int q = 2 * KB - 1; q > 0; q--) xw[q] = xw[q - 1];
+            xw[0] = x;
         }
     }
 }
@@ -316,11 +356,14 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
     sc.site_count[site] = 0;
 }
 
-void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n, int nsys,
-                     int KB, double* band, const long long* band_off, int* outside, double* ywork, hipStream_t s) {
+void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
+                     const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
+                     double* ywork, hipStream_t s) {
     if (nsys <= 0) return;
-    k_eps_band_assemble<<<nsys, 256, 0, s>>>(m, g, sys_first, sys_n, KB, band, band_off, outside);
-    k_eps_band_solve<<<nsys, 64, 0, s>>>(sys_first, sys_n, outside, KB, band, band_off, ywork, m.eps);
+    k_eps_band_assemble<<<nsys, 256, sizeof(double) * 2 * kEpsLdsMarkers, s>>>(m, g, sys_first, sys_n, KB, band,
+                                                                                band_off, outside);
+    if (KB != 8) return;  // the register window is built for KB = 8
+    k_eps_band_solve<8><<<nsys, 64, 0, s>>>(sys_first, sys_n, outside, band, band_off, ywork, m.eps);
 }
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
@@ -328,9 +371,9 @@ void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_
     k_find_support<<<(m.n + 127) / 128, 128, 0, s>>>(m, g, flexible_only);
 }
 
-void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
-    if (m.n <= 0) return;
-    k_compute_ds<<<(m.n + 127) / 128, 128, 0, s>>>(m, g, flexible_only);
+void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s) {
+    if (m.n <= 0 || nbodies <= 0) return;
+    k_compute_ds<<<nbodies, 256, sizeof(double) * 2 * kDsLdsMarkers, s>>>(m, g, flexible_only);
 }
 
 void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s) {

@@ -82,13 +82,14 @@ struct SpreadScratch {
 };
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s);
-void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s);
+void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s);
 void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s);
 // Banded epsilon systems (one per entry of sys_first/sys_n), assembled and
 // solved on the device into m.eps; outside[k] = 1 marks a system whose
 // matrix is not confined to the band (left for the dense host path).
-void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n, int nsys,
-                     int KB, double* band, const long long* band_off, int* outside, double* ywork, hipStream_t s);
+void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
+                     const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
+                     double* ywork, hipStream_t s);
 template <typename T>
 void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s);
 void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,

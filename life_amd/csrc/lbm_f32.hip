@@ -12,12 +12,17 @@
 // (lbm_f64.hip).  72 B of HBM traffic per site update.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "dev_util.h"
 #include "launch.h"
 
 namespace life {
 
 #define LIFE_DF __device__ __forceinline__
+
+using v4f = float __attribute__((ext_vector_type(4)));
 
 LIFE_DF float w32(int v) { return v == 0 ? 4.0f / 9.0f : (v < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
 
@@ -120,18 +125,30 @@ LIFE_DF void moments32(const float gs[kNV], float Fx, float Fy, float& drho, flo
     uy = (jy + 0.5f * Fy) * inv;
 }
 
-template <bool CM, bool EXPLICIT>
+// Store with or without the non-temporal (streaming) hint.
+template <bool NT>
+LIFE_DF void st32(float* p, float v) {
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// One site per lane.  NT selects the streaming hint: bit 0 for the loads,
+// bit 1 for the stores that stay in their row and column (c = 0, +-x),
+// bit 2 for the stores shifted by one site along the row.  XCD = remap
+// blocks to XCD-contiguous ranges.
+template <bool CM, bool EXPLICIT, int NT = 3, bool XCD = true>
 __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, float* __restrict__ dst, GridScalars g,
                                                 const double* __restrict__ rho0, const double* __restrict__ u0,
                                                 const double* __restrict__ fibm, unsigned nblocks) {
-    const long long id = (long long)xcd_block(blockIdx.x, nblocks) * blockDim.x + threadIdx.x;
+    const unsigned blk = XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x;
+    const long long id = (long long)blk * blockDim.x + threadIdx.x;
     if (id >= g.N) return;
     const long long S = g.S;
     const int Ny = g.Ny, Nx = g.Nx;
     const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
     float gs[kNV], go[kNV];
 #pragma unroll
-    for (int v = 0; v < kNV; v++) gs[v] = __builtin_nontemporal_load(&src[v * S + id]);
+    for (int v = 0; v < kNV; v++) gs[v] = (NT & 1) ? __builtin_nontemporal_load(&src[v * S + id]) : src[v * S + id];
     float drho, ux, uy, Fx = (float)g.Fx, Fy = (float)g.Fy;
     if (EXPLICIT) {
         drho = (float)(rho0[id] - 1.0);
@@ -146,15 +163,103 @@ __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, f
     const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
     const int jm = j == 0 ? Ny - 1 : j - 1, jp = j == Ny - 1 ? 0 : j + 1;
     const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
-    __builtin_nontemporal_store(go[0], &dst[0 * S + r0 + j]);
-    __builtin_nontemporal_store(go[1], &dst[1 * S + rp + j]);
-    __builtin_nontemporal_store(go[2], &dst[2 * S + rm + j]);
-    dst[3 * S + r0 + jp] = go[3];
-    dst[4 * S + r0 + jm] = go[4];
-    dst[5 * S + rp + jp] = go[5];
-    dst[6 * S + rm + jm] = go[6];
-    dst[7 * S + rp + jm] = go[7];
-    dst[8 * S + rm + jp] = go[8];
+    constexpr bool A = (NT & 2) != 0, B = (NT & 4) != 0;
+    st32<A>(&dst[0 * S + r0 + j], go[0]);
+    st32<A>(&dst[1 * S + rp + j], go[1]);
+    st32<A>(&dst[2 * S + rm + j], go[2]);
+    st32<B>(&dst[3 * S + r0 + jp], go[3]);
+    st32<B>(&dst[4 * S + r0 + jm], go[4]);
+    st32<B>(&dst[5 * S + rp + jp], go[5]);
+    st32<B>(&dst[6 * S + rm + jm], go[6]);
+    st32<B>(&dst[7 * S + rp + jm], go[7]);
+    st32<B>(&dst[8 * S + rm + jp], go[8]);
+}
+
+// Vector form of k_push32 for Ny % 4 == 0: each lane owns four consecutive
+// sites of one row and moves each population plane with one aligned 16-byte
+// load and one aligned 16-byte store.  Populations with c_y = +-1 land one
+// site up/down the row; the element that crosses into the neighbouring
+// lane's float4 is exchanged with a wave shuffle.  A lane at the start of a
+// wave or row ("head") and one at the end ("tail") cannot complete that
+// exchange and write the affected elements one by one instead, including
+// the periodic wrap of the row (Grid.cpp:229, 240).
+template <bool CM>
+__global__ __launch_bounds__(256) void k_push32x4(const float* __restrict__ src, float* __restrict__ dst,
+                                                  GridScalars g, unsigned nblocks) {
+    const long long q = (long long)xcd_block(blockIdx.x, nblocks) * blockDim.x + threadIdx.x;
+    const long long id0 = 4 * q;
+    const bool valid = id0 < g.N;
+    const long long S = g.S;
+    const int Ny = g.Ny, Nx = g.Nx;
+    const int lane = threadIdx.x & 63;
+    const long long idc = valid ? id0 : 0;
+    const int i = (int)(idc / Ny), j0 = (int)(idc - (long long)i * Ny);
+    v4f in[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++)
+        in[v] = valid ? __builtin_nontemporal_load(reinterpret_cast<const v4f*>(src + v * S + idc))
+                      : v4f{0.f, 0.f, 0.f, 0.f};
+    float o[kNV][4];
+    const float Fx = (float)g.Fx, Fy = (float)g.Fy;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        float gs[kNV], go[kNV];
+#pragma unroll
+        for (int v = 0; v < kNV; v++) gs[v] = s == 0 ? in[v].x : (s == 1 ? in[v].y : (s == 2 ? in[v].z : in[v].w));
+        float drho, ux, uy;
+        moments32(gs, Fx, Fy, drho, ux, uy);
+        collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
+#pragma unroll
+        for (int v = 0; v < kNV; v++) o[v][s] = go[v];
+    }
+    const bool head = lane == 0 || j0 == 0;
+    const bool tail = lane == 63 || j0 + 4 == Ny;
+    const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
+    const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
+    // c_y = 0: whole float4 to the same j
+    if (valid) {
+        *reinterpret_cast<v4f*>(dst + 0 * S + r0 + j0) = v4f{o[0][0], o[0][1], o[0][2], o[0][3]};
+        *reinterpret_cast<v4f*>(dst + 1 * S + rp + j0) = v4f{o[1][0], o[1][1], o[1][2], o[1][3]};
+        *reinterpret_cast<v4f*>(dst + 2 * S + rm + j0) = v4f{o[2][0], o[2][1], o[2][2], o[2][3]};
+    }
+    // c_y = +1 (v = 3, 5, 8): site j -> j+1
+    const int vup[3] = {3, 5, 8};
+    const long long rup[3] = {r0, rp, rm};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int v = vup[k];
+        const float prev = __shfl_up(o[v][3], 1);
+        float* base = dst + v * S + rup[k];
+        if (valid) {
+            if (head) {
+                base[j0 + 1] = o[v][0];
+                base[j0 + 2] = o[v][1];
+                base[j0 + 3] = o[v][2];
+            } else {
+                *reinterpret_cast<v4f*>(base + j0) = v4f{prev, o[v][0], o[v][1], o[v][2]};
+            }
+            if (tail) base[j0 + 4 == Ny ? 0 : j0 + 4] = o[v][3];
+        }
+    }
+    // c_y = -1 (v = 4, 6, 7): site j -> j-1
+    const int vdn[3] = {4, 6, 7};
+    const long long rdn[3] = {r0, rm, rp};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int v = vdn[k];
+        const float next = __shfl_down(o[v][0], 1);
+        float* base = dst + v * S + rdn[k];
+        if (valid) {
+            if (tail) {
+                base[j0] = o[v][1];
+                base[j0 + 1] = o[v][2];
+                base[j0 + 2] = o[v][3];
+            } else {
+                *reinterpret_cast<v4f*>(base + j0) = v4f{o[v][1], o[v][2], o[v][3], next};
+            }
+            if (head) base[j0 == 0 ? Ny - 1 : j0 - 1] = o[v][0];
+        }
+    }
 }
 
 template <bool CM>
@@ -183,16 +288,67 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
     }
 }
 
+// Launch configuration of the bulk stream-collide kernel, read once from
+// the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 (kernel),
+// LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32), LIFE_PUSH32_XCD
+// = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256 (block size).
+struct PushConfig {
+    bool x4 = false;
+    int nt = 3;
+    bool xcd = true;
+    int bs = 256;
+    PushConfig() {
+        if (const char* e = std::getenv("LIFE_PUSH32")) x4 = std::strcmp(e, "x4") == 0;
+        if (const char* e = std::getenv("LIFE_PUSH32_NT")) nt = std::atoi(e) & 7;
+        if (const char* e = std::getenv("LIFE_PUSH32_XCD")) xcd = std::atoi(e) != 0;
+        if (const char* e = std::getenv("LIFE_PUSH32_BS")) bs = std::atoi(e);
+        if (bs != 64 && bs != 128 && bs != 256) bs = 256;
+    }
+};
+
+static const PushConfig& push_config() {
+    static const PushConfig c;
+    return c;
+}
+
+template <bool CM, int NT>
+static void launch_push32_nt(const float* src, float* dst, const GridScalars& g, const PushConfig& c, hipStream_t s) {
+    const unsigned nb = (unsigned)((g.N + c.bs - 1) / c.bs);
+    if (c.xcd) k_push32<CM, false, NT, true><<<nb, c.bs, 0, s>>>(src, dst, g, nullptr, nullptr, nullptr, nb);
+    else k_push32<CM, false, NT, false><<<nb, c.bs, 0, s>>>(src, dst, g, nullptr, nullptr, nullptr, nb);
+}
+
+template <bool CM>
+static void launch_push32_bulk(const float* src, float* dst, const GridScalars& g, hipStream_t s) {
+    const PushConfig& c = push_config();
+    if (c.x4 && g.Ny % 4 == 0) {
+        const long long nq = g.N / 4;
+        const unsigned nb = (unsigned)((nq + 255) / 256);
+        k_push32x4<CM><<<nb, 256, 0, s>>>(src, dst, g, nb);
+        return;
+    }
+    switch (c.nt) {
+        case 0: launch_push32_nt<CM, 0>(src, dst, g, c, s); break;
+        case 1: launch_push32_nt<CM, 1>(src, dst, g, c, s); break;
+        case 2: launch_push32_nt<CM, 2>(src, dst, g, c, s); break;
+        case 3: launch_push32_nt<CM, 3>(src, dst, g, c, s); break;
+        case 4: launch_push32_nt<CM, 4>(src, dst, g, c, s); break;
+        case 5: launch_push32_nt<CM, 5>(src, dst, g, c, s); break;
+        case 6: launch_push32_nt<CM, 6>(src, dst, g, c, s); break;
+        default: launch_push32_nt<CM, 7>(src, dst, g, c, s); break;
+    }
+}
+
 void launch_push32(const float* src, float* dst, const GridScalars& g, const double* rho0, const double* u0,
                    const double* fibm, bool explicit_state, hipStream_t s) {
-    const unsigned nb = (unsigned)((g.N + 255) / 256);
-    if (g.cm) {
-        if (explicit_state) k_push32<true, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
-        else k_push32<true, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
-    } else {
-        if (explicit_state) k_push32<false, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
-        else k_push32<false, false><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    if (!explicit_state) {
+        if (g.cm) launch_push32_bulk<true>(src, dst, g, s);
+        else launch_push32_bulk<false>(src, dst, g, s);
+        return;
     }
+    const unsigned nb = (unsigned)((g.N + 255) / 256);
+    if (g.cm) k_push32<true, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
+    else k_push32<false, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
 }
 
 void launch_support_push32(const float* src, float* dst, const GridScalars& g, const double* fibm, const int* sites,

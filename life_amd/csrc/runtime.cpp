@@ -153,7 +153,7 @@ void ibm_spread(life_grid* g) {
 void ibm_find_support(life_grid* g, int flexible_only) {
     if (g->md.n <= 0) return;
     launch_find_support(g->md, g->g, flexible_only, g->stream);
-    launch_compute_ds(g->md, g->g, flexible_only, g->stream);
+    launch_compute_ds(g->md, g->g, g->nbodies, flexible_only, g->stream);
     LIFE_HIP(hipGetLastError());
 }
 
@@ -174,6 +174,7 @@ void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const s
     for (void* p : {(void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
                     (void*)g->eps_band, (void*)g->eps_y})
         dfree(p);
+    if (KB != 8) throw std::runtime_error("banded epsilon solve is built for a half-bandwidth of 8");
     g->eps_nsys = (int)first.size();
     g->eps_KB = KB;
     g->eps_first_h = first;
@@ -204,7 +205,7 @@ std::vector<int> ibm_epsilon_band(life_grid* g) {
     std::vector<int> dense;
     if (g->eps_nsys <= 0) return dense;
     launch_fill_i32(g->eps_outside, 0, g->eps_nsys, g->stream);
-    launch_eps_band(g->md, g->g, g->eps_first, g->eps_n, g->eps_nsys, g->eps_KB, g->eps_band, g->eps_off,
+    launch_eps_band(g->md, g->g, g->eps_first, g->eps_n, g->eps_n_h.data(), g->eps_nsys, g->eps_KB, g->eps_band, g->eps_off,
                     g->eps_outside, g->eps_y, g->stream);
     LIFE_HIP(hipGetLastError());
     std::vector<int> out(g->eps_nsys);

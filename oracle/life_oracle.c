@@ -1539,6 +1539,80 @@ void orc_coupling_info(const orc_case *c, double *out) {
     out[2] = c->relax;
 }
 
+/* A clamped horizontal filament FEM body on its own (no fluid, no
+ * markers), for checking the beam against closed forms: with nsteps == 0,
+ * the static tip deflection under a tip load P through the static Newton
+ * iteration of FEMBodyClass::staticFEM (FEMBody.cpp:437-484); with
+ * nsteps > 0, the tip y displacement after releasing that static state,
+ * one Newmark step (FEMBody.cpp:26-144 without the IBM load) per entry. */
+int orc_fem_cantilever(double L, double h, double E, double rho, int nel, double Dt, double P, int nsteps,
+                       double *tip_out) {
+    init_tables();
+    orc_case c;
+    memset(&c, 0, sizeof c);
+    c.Dx = 1.0;
+    c.Dt = Dt;
+    c.Dm = 1.0;
+    c.p.alpha = 0.25;
+    c.p.delta = 0.5;
+    c.p.ref_L = L;
+    body_t b;
+    memset(&b, 0, sizeof b);
+    b.n = nel + 1;
+    char nels[32];
+    snprintf(nels, sizeof nels, "%d", nel);
+    double pos[2] = {0.0, 0.0}, geom[2] = {L, h};
+    fem_t *F = fem_create(&c, &b, pos, geom, 0.0, nels, "CLAMPED", rho, E);
+    int nd = F->ndof;
+    /* static Newton under the tip load */
+    F->R[nd - 3 + 1] = P;
+    double *rhs = xcalloc(nd, sizeof(double));
+    int it = 0;
+    double res;
+    do {
+        fem_build(F);
+        for (int k = 0; k < nd; k++) rhs[k] = F->R[k] - F->F[k];
+        orc_lu_solve(nd, F->nbc, F->K, rhs, F->delU);
+        for (int k = 0; k < nd; k++) F->U[k] = F->U[k] + F->delU[k];
+        fem_update_fem(F);
+        res = sqrt(vdot(F->delU, F->delU, nd)) / (L * sqrt((double)nd));
+        it++;
+    } while (res > 1e-12 && it < 100);
+    if (nsteps == 0) {
+        tip_out[0] = F->U[nd - 3 + 1];
+        free(rhs);
+        fem_free(F);
+        return it;
+    }
+    /* release: R = 0, consistent initial acceleration M a = -F_int */
+    memset(F->R, 0, sizeof(double) * nd);
+    fem_build(F);
+    for (int k = 0; k < nd; k++) rhs[k] = -F->F[k];
+    orc_lu_solve(nd, F->nbc, F->M, rhs, F->Uddn);
+    memcpy(F->Un, F->U, sizeof(double) * nd);
+    memset(F->Udn, 0, sizeof(double) * nd);
+    for (int s = 0; s < nsteps; s++) {
+        memcpy(F->U, F->Un, sizeof(double) * nd);
+        memcpy(F->Ud, F->Udn, sizeof(double) * nd);
+        memcpy(F->Udd, F->Uddn, sizeof(double) * nd);
+        fem_update_fem(F);
+        int k = 0;
+        do {
+            fem_newton_dynamic(&c, F);
+            res = sqrt(vdot(F->delU, F->delU, nd)) / (L * sqrt((double)nd));
+            k++;
+        } while (res > 1e-12 && k < 20);
+        fem_finish_newmark(&c, F);
+        tip_out[s] = F->U[nd - 3 + 1];
+        memcpy(F->Un, F->U, sizeof(double) * nd);
+        memcpy(F->Udn, F->Ud, sizeof(double) * nd);
+        memcpy(F->Uddn, F->Udd, sizeof(double) * nd);
+    }
+    free(rhs);
+    fem_free(F);
+    return 0;
+}
+
 /* ObjectsClass::writeTotalForces, Objects.cpp:517-534. */
 void orc_total_force(const orc_case *c, double out[4]) {
     double scale = c->Dm * c->Dx / (c->Dt * c->Dt) * 1.0 / c->Dx;

@@ -84,6 +84,10 @@ int orc_lu_solve(int n, int bc, const double *A, const double *b, double *x);
 void orc_collide_site(int cm, double omega, const double *fsite, double rho, double ux, double uy,
                       double Fx, double Fy, double *out);
 void orc_equilibrium(int cm, double rho, double ux, double uy, double *feq);
+/* Clamped horizontal beam alone: static tip deflection under tip load P
+ * (nsteps == 0, tip_out[0]) or tip displacement history after release. */
+int orc_fem_cantilever(double L, double h, double E, double rho, int nel, double Dt, double P, int nsteps,
+                       double *tip_out);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,8 @@ def lib():
                                        C.c_double, C.c_double, dp]
         L.orc_equilibrium.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_fem_cantilever.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                         C.c_double, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -244,6 +246,14 @@ def collide_site(cm, omega, fsite, rho, ux, uy, Fx=0.0, Fy=0.0):
     lib().orc_collide_site(int(cm), float(omega), _dp(fs), float(rho), float(ux), float(uy),
                            float(Fx), float(Fy), _dp(out))
     return out
+
+
+def fem_cantilever(L, h, E, rho, nel, Dt, P, nsteps=0):
+    """Static tip deflection (nsteps=0) or tip history after release."""
+    out = np.zeros(max(1, nsteps))
+    lib().orc_fem_cantilever(float(L), float(h), float(E), float(rho), int(nel), float(Dt), float(P),
+                             int(nsteps), _dp(out))
+    return out[0] if nsteps == 0 else out
 
 
 def equilibrium(cm, rho, ux, uy):
