@@ -62,16 +62,19 @@ __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
         for (int t = threadIdx.x; t < 2 * cnt; t += blockDim.x) spos[t] = m.pos[2 * first + t];
     __syncthreads();
     const double* P = lds ? spos : m.pos + 2 * first;
+    // sqrt(.)/Dx is monotone, so the minimum of sqrt(d.d)/Dx over the other
+    // markers is sqrt(min d.d)/Dx: one square root per marker, same value
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
         const double x = P[2 * k], y = P[2 * k + 1];
-        double ds = 10.0;
+        double d2min = INFINITY;
         for (int q = 0; q < cnt; q++) {
             if (q == k) continue;
             const double dx = x - P[2 * q], dy = y - P[2 * q + 1];
-            const double mag = sqrt(0.0 + dx * dx + dy * dy) / g.Dx;
-            if (mag < ds) ds = mag;
+            const double d2 = 0.0 + dx * dx + dy * dy;
+            if (d2 < d2min) d2min = d2;
         }
-        m.ds[first + k] = ds;
+        const double mag = sqrt(d2min) / g.Dx;
+        m.ds[first + k] = mag < 10.0 ? mag : 10.0;
     }
 }
 
@@ -96,56 +99,55 @@ __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, d
 }
 
 // ---------------------------------------------------------------- epsilon, banded
-constexpr int kEpsLdsMarkers = 2048;  // systems up to this size stage positions in LDS
 // Batched form of computeEpsilon for systems whose matrix is banded (every
 // filament: A_ij = 0 unless markers i and j are within 3 lattice spacings).
-// One workgroup per system assembles rows i, columns |i-j| <= KB into band
-// storage (row width 3*KB+1: room for the fill-in of partial pivoting) and
-// flags the system if any pair outside the band is close enough to couple;
-// flagged systems are solved on the host with the dense path instead.
-__global__ void k_eps_band_assemble(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
-                                    const int* __restrict__ sys_n, int KB, double* __restrict__ band,
-                                    const long long* __restrict__ band_off, int* __restrict__ outside) {
-    const int sys = blockIdx.x;
+// Band storage: row r holds columns r-KB .. r+2KB (width 3KB+1, the upper
+// part leaves room for the fill-in of partial pivoting).  A system with a
+// coupling pair outside the band is flagged and solved on the host with the
+// dense path instead.
+
+// One thread per band entry; grid.y = system.
+__global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                   const int* __restrict__ sys_n, int KB, double* __restrict__ band,
+                                   const long long* __restrict__ band_off) {
+    const int sys = blockIdx.y;
     const int first = sys_first[sys], n = sys_n[sys];
     const int W = 3 * KB + 1;
-    double* B = band + band_off[sys];
-    const double Dx = g.Dx;
-    // band entries (and zero fill-in columns)
-    for (long long t = threadIdx.x; t < (long long)n * W; t += blockDim.x) {
-        const int r = (int)(t / W), q = (int)(t - (long long)r * W);
-        const int c = r - KB + q;
-        double a = 0.0;
-        if (q <= 2 * KB && c >= 0 && c < n) {
-            const int mi = first + r, mj = first + c;
-            const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
-            const int cnt = m.supp_cnt[mi];
-            for (int s = 0; s < cnt; s++) {
-                const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
-                const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
-                a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
-            }
-            a = a * m.ds[mj];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * W) return;
+    const int r = (int)(t / W), q = (int)(t - (long long)r * W);
+    const int c = r - KB + q;
+    double a = 0.0;
+    if (q <= 2 * KB && c >= 0 && c < n) {
+        const int mi = first + r, mj = first + c;
+        const double Dx = g.Dx;
+        const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
+        const int cnt = m.supp_cnt[mi];
+        for (int s = 0; s < cnt; s++) {
+            const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
+            const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
+            a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
         }
-        B[t] = a;
+        a = a * m.ds[mj];
     }
-    // pairs outside the band that could couple (support of i within 1.5 of
-    // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings);
-    // the test is symmetric, so only c > r + KB is visited
-    extern __shared__ double spos[];
-    const bool lds = n <= kEpsLdsMarkers;
-    if (lds)
-        for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) spos[t] = m.pos[2 * first + t];
-    __syncthreads();
-    const double* P = lds ? spos : m.pos + 2 * first;
-    const double lim = 3.01 * Dx;
-    int far = 0;
-    for (int r = threadIdx.x / 64; r < n && !far; r += blockDim.x / 64) {
-        const double xr = P[2 * r], yr = P[2 * r + 1];
-        for (int c = r + KB + 1 + (threadIdx.x & 63); c < n; c += 64)
-            if (fabs(xr - P[2 * c]) < lim && fabs(yr - P[2 * c + 1]) < lim) far = 1;
-    }
-    if (far) outside[sys] = 1;
+    band[band_off[sys] + t] = a;
+}
+
+// Pairs outside the band that could couple (support of i within 1.5 of
+// x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings).
+// One thread per pair r < c - KB (the test is symmetric); grid.y = system.
+__global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                const int* __restrict__ sys_n, int KB, int* __restrict__ outside) {
+    const int sys = blockIdx.y;
+    const int first = sys_first[sys], n = sys_n[sys];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * n) return;
+    const int r = (int)(t / n), c = (int)(t - (long long)r * n);
+    if (c <= r + KB) return;
+    const double lim = 3.01 * g.Dx;
+    const int mi = first + r, mj = first + c;
+    if (fabs(m.pos[2 * mi] - m.pos[2 * mj]) < lim && fabs(m.pos[2 * mi + 1] - m.pos[2 * mj + 1]) < lim)
+        outside[sys] = 1;
 }
 
 // Canonical elimination (partial pivoting, first maximum) of one banded
@@ -160,23 +162,33 @@ __global__ void k_eps_band_assemble(MarkerDev m, GridScalars g, const int* __res
 // operands as the dense algorithm), emits row k of U, and slides the window
 // one row and one column.  Back substitution then runs on one lane in
 // increasing column order.  The skipped dense operations all involve an
-// exact zero, so the values equal the dense elimination's.
+// exact zero, so the values equal the dense elimination's.  The band is
+// staged in LDS (rows of U replace rows of the band as they are emitted)
+// when it fits in lds_rows rows.
 template <int KB>
 __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
                                                        const int* __restrict__ outside, double* __restrict__ band,
                                                        const long long* __restrict__ band_off,
-                                                       double* __restrict__ ywork, double* __restrict__ eps) {
+                                                       double* __restrict__ ywork, double* __restrict__ eps,
+                                                       int lds_rows) {
     constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
+    extern __shared__ double sm[];
     const int sys = blockIdx.x;
     if (outside[sys]) return;
     const int first = sys_first[sys], n = sys_n[sys];
-    double* B = band + band_off[sys];
-    double* y = ywork + first;
+    const bool in_lds = n <= lds_rows;
+    double* G = band + band_off[sys];
+    double* B = in_lds ? sm : G;
+    double* y = in_lds ? sm + (long long)n * W : ywork + first;
     const int c = threadIdx.x;
+    if (in_lds) {
+        for (int t = c; t < n * W; t += 64) B[t] = G[t];
+        __syncthreads();
+    }
     auto A0 = [&](int r, int col) -> double {
         if (c >= C || r >= n || col < 0 || col >= n) return 0.0;
         const int q = col - r + KB;
-        return (q >= 0 && q < W) ? B[(long long)r * W + q] : 0.0;
+        return (q >= 0 && q < W) ? B[r * W + q] : 0.0;
     };
     double w[R], yw[R];
 #pragma unroll
@@ -185,6 +197,7 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
         yw[r] = r < n ? 1.0 : 0.0;
     }
     for (int k = 0; k < n; k++) {
+        const double incoming = A0(k + R, k + 1 + c);  // row entering the window next
         const int rmax = min(KB, n - 1 - k);
         double col[R];
 #pragma unroll
@@ -202,7 +215,7 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
             }
         if (best != 0.0) {
             if (p != 0) {
-                double tw = w[0], tc = col[0], ty = yw[0];
+                const double tw = w[0], tc = col[0], ty = yw[0];
 #pragma unroll
                 for (int r = 1; r < R; r++)
                     if (r == p) {
@@ -225,7 +238,7 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
                 }
         }
         // row k of U (in place of row k of the band) and the final y_k
-        if (c < C && k + c < n) B[(long long)k * W + (c + KB)] = w[0];
+        if (c < C && k + c < n) B[k * W + (c + KB)] = w[0];
         if (c == 0) y[k] = yw[0];
         // slide: drop row k and column k, take in row k+R
 #pragma unroll
@@ -233,15 +246,16 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
             w[r] = __shfl_down(w[r + 1], 1);
             yw[r] = yw[r + 1];
         }
-        w[R - 1] = A0(k + R, k + 1 + c);
+        w[R - 1] = incoming;
         yw[R - 1] = k + R < n ? 1.0 : 0.0;
     }
+    __syncthreads();
     if (c == 0) {
         double xw[2 * KB];
 #pragma unroll
         for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
         for (int i = n - 1; i >= 0; i--) {
-            const double* Ui = B + (long long)i * W + KB;
+            const double* Ui = B + i * W + KB;
             double s = y[i];
 #pragma unroll
             for (int q = 0; q < 2 * KB; q++)
@@ -359,11 +373,19 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
 void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
                      const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
                      double* ywork, hipStream_t s) {
-    if (nsys <= 0) return;
-    k_eps_band_assemble<<<nsys, 256, sizeof(double) * 2 * kEpsLdsMarkers, s>>>(m, g, sys_first, sys_n, KB, band,
-                                                                                band_off, outside);
-    if (KB != 8) return;  // the register window is built for KB = 8
-    k_eps_band_solve<8><<<nsys, 64, 0, s>>>(sys_first, sys_n, outside, band, band_off, ywork, m.eps);
+    if (nsys <= 0 || KB != 8) return;
+    int max_n = 0;
+    for (int k = 0; k < nsys; k++) max_n = max_n > host_n[k] ? max_n : host_n[k];
+    const int W = 3 * KB + 1;
+    const long long ne = (long long)max_n * W, np = (long long)max_n * max_n;
+    k_eps_band_entries<<<dim3((unsigned)((ne + 255) / 256), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, band,
+                                                                                 band_off);
+    k_eps_far_check<<<dim3((unsigned)((np + 255) / 256), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, outside);
+    // band, then right-hand side, in LDS for systems of up to lds_rows rows
+    const int lds_rows = (int)((60 * 1024) / (sizeof(double) * (W + 1)));
+    const int rows = max_n < lds_rows ? max_n : lds_rows;
+    k_eps_band_solve<8><<<nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(sys_first, sys_n, outside, band,
+                                                                                   band_off, ywork, m.eps, rows);
 }
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {

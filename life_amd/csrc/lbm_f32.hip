@@ -175,6 +175,29 @@ __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, f
     st32<B>(&dst[8 * S + rm + jp], go[8]);
 }
 
+// Bulk stream-collide with LDS-realigned stores (see push_store_realigned).
+template <bool CM, bool XCD = true>
+__global__ __launch_bounds__(kPushBlock) void k_push32_lds(const float* __restrict__ src, float* __restrict__ dst,
+                                                           GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<float> st;
+    const long long base = (long long)(XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x) * kPushBlock;
+    const int t = threadIdx.x;
+    const long long id = base + t;
+    const bool valid = id < g.N;
+    const long long idc = valid ? id : 0;
+    const long long S = g.S;
+    const int Ny = g.Ny;
+    const int i = (int)(idc / Ny), j = (int)(idc - (long long)i * Ny);
+    float gs[kNV], go[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++) gs[v] = valid ? __builtin_nontemporal_load(&src[v * S + idc]) : 0.0f;
+    float drho, ux, uy;
+    const float Fx = (float)g.Fx, Fy = (float)g.Fy;
+    moments32(gs, Fx, Fy, drho, ux, uy);
+    collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
+    push_store_realigned<float, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, go, valid, st);
+}
+
 // Vector form of k_push32 for Ny % 4 == 0: each lane owns four consecutive
 // sites of one row and moves each population plane with one aligned 16-byte
 // load and one aligned 16-byte store.  Populations with c_y = +-1 land one
@@ -294,11 +317,15 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
 // = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256 (block size).
 struct PushConfig {
     bool x4 = false;
+    bool lds = true;
     int nt = 3;
     bool xcd = true;
     int bs = 256;
     PushConfig() {
-        if (const char* e = std::getenv("LIFE_PUSH32")) x4 = std::strcmp(e, "x4") == 0;
+        if (const char* e = std::getenv("LIFE_PUSH32")) {
+            x4 = std::strcmp(e, "x4") == 0;
+            lds = std::strcmp(e, "lds") == 0;
+        }
         if (const char* e = std::getenv("LIFE_PUSH32_NT")) nt = std::atoi(e) & 7;
         if (const char* e = std::getenv("LIFE_PUSH32_XCD")) xcd = std::atoi(e) != 0;
         if (const char* e = std::getenv("LIFE_PUSH32_BS")) bs = std::atoi(e);
@@ -321,6 +348,12 @@ static void launch_push32_nt(const float* src, float* dst, const GridScalars& g,
 template <bool CM>
 static void launch_push32_bulk(const float* src, float* dst, const GridScalars& g, hipStream_t s) {
     const PushConfig& c = push_config();
+    if (c.lds) {
+        const unsigned nb = (unsigned)((g.N + kPushBlock - 1) / kPushBlock);
+        if (c.xcd) k_push32_lds<CM, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        else k_push32_lds<CM, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        return;
+    }
     if (c.x4 && g.Ny % 4 == 0) {
         const long long nq = g.N / 4;
         const unsigned nb = (unsigned)((nq + 255) / 256);

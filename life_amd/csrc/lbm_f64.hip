@@ -12,6 +12,9 @@
 // Compiled with -ffp-contract=off: bit-compatible with the reference.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "d2q9_f64.h"
 #include "dev_util.h"
 #include "launch.h"
@@ -56,6 +59,29 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
     dst[8 * S + rm + jp] = fo[8];
 }
 
+// Bulk step with LDS-realigned stores (push_store_realigned): the same
+// values reach the same addresses as k_push64, with aligned wavefront stores.
+template <bool CM>
+__global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
+                                                           GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<double> st;
+    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * kPushBlock;
+    const int t = threadIdx.x;
+    const long long id = base + t;
+    const bool valid = id < g.N;
+    const long long idc = valid ? id : 0;
+    const long long S = g.S;
+    const int Ny = g.Ny;
+    const int i = (int)(idc / Ny), j = (int)(idc - (long long)i * Ny);
+    double fs[kNV], fo[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++) fs[v] = valid ? __builtin_nontemporal_load(&src[v * S + idc]) : 1.0;
+    double rho, ux, uy;
+    moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+    collide64<CM>(g, fs, rho, ux, uy, g.Fx0, g.Fy0, fo);
+    push_store_realigned<double, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
+}
+
 // Redo the push of the support sites of the last spread, where the stored
 // u includes force_ibm and the Guo/central-moment forcing uses
 // force_xy + force_ibm.  sites[0..*count) are lattice ids.
@@ -87,6 +113,16 @@ __global__ __launch_bounds__(256) void k_support_push64(const double* __restrict
 
 void launch_push64(const double* src, double* dst, const GridScalars& g, const double* rho0, const double* u0,
                    const double* fibm, bool explicit_state, hipStream_t s) {
+    static const bool lds = [] {
+        const char* e = std::getenv("LIFE_PUSH64");
+        return !(e && std::strcmp(e, "scalar") == 0);
+    }();
+    if (!explicit_state && lds) {
+        const unsigned nb = (unsigned)((g.N + kPushBlock - 1) / kPushBlock);
+        if (g.cm) k_push64_lds<true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        else k_push64_lds<false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        return;
+    }
     const unsigned nb = (unsigned)((g.N + 255) / 256);
     if (g.cm) {
         if (explicit_state) k_push64<true, true><<<nb, 256, 0, s>>>(src, dst, g, rho0, u0, fibm, nb);
