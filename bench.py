@@ -12,6 +12,15 @@ wall time), the roofline of the fused stream-collide kernel (72 algorithmic
 bytes per site update over its HIP-event-timed average launch), the FP32 vs
 FP64 velocity error on the same run, and a CPU baseline (the oracle, FP64,
 OpenMP, on a bounded number of steps of the same workload).
+
+--workload serve measures the serving engine instead: a seeded Poisson
+stream of independent inverted-flag cases (arrivals, sizes, Re, stiffness
+and element counts drawn as described in life_amd.config.serving_stream)
+run by the persistent request loop with FP32-shifted populations, reporting
+cases/hour, p50/p99 time-to-solution (arrival to finish) and aggregate
+MLUPS; beside it, a prefix of the same stream is run both by the engine and
+by the plain FP64 path one case at a time in arrival order, and the FP32
+results are compared with the FP64 ones (tip position, force).
 """
 import argparse
 import json
@@ -82,6 +91,108 @@ def calibrate_copy(nbytes, reps):
     print("copied %d bytes x %d" % (nbytes, reps))
 
 
+def serve_summary(srv, ids, wall):
+    import numpy as np
+    from life_amd import lib
+    infos = [srv.info(i) for i in ids]
+    done = [x for x in infos if x["status"] == lib.LIFE_REQ_DONE]
+    tts = np.array([x["t_finish"] - x["t_arrival"] for x in done]) if done else np.zeros(1)
+    st = srv.stats()
+    return {"cases": len(infos), "done": len(done), "wall_s": wall,
+            "cases_per_hour": len(done) / wall * 3600.0,
+            "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
+            "mean_s": float(tts.mean()), "mlups": st["site_updates"] / wall / 1e6,
+            "mlups_busy": st["site_updates"] / max(st["busy_s"], 1e-9) / 1e6,
+            "engine_steps": st["steps"], "ms_per_engine_step": st["busy_s"] * 1e3 / max(st["steps"], 1.0),
+            "graph_captures": st["graph_captures"], "peak_running": st["peak_running"],
+            "peak_bytes": st["peak_bytes"]}, infos
+
+
+def run_one_at_a_time(stream, prec):
+    """Plain path, one case at a time in arrival order: case k starts at
+    max(its arrival, the previous finish)."""
+    import numpy as np
+    from life_amd.case import Case
+    t0 = time.perf_counter()
+    out = []
+    for r in stream:
+        now = time.perf_counter() - t0
+        if now < r["arrival"]:
+            time.sleep(r["arrival"] - now)
+        c = Case(r["params"], r["geometry"], precision=prec)
+        c.step(r["steps"])
+        m = c.markers()["pos"].reshape(-1, 2)
+        out.append({"t_finish": time.perf_counter() - t0, "t_arrival": r["arrival"], "tip": m[-1].copy(),
+                    "force": c.total_force()[:2].copy(), "L": r["params"].ref_L})
+        c.close()
+        print("bench: one-at-a-time case %d/%d done" % (len(out), len(stream)), file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    tts = np.array([x["t_finish"] - x["t_arrival"] for x in out])
+    return {"cases": len(out), "wall_s": wall, "cases_per_hour": len(out) / wall * 3600.0,
+            "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
+            "mlups": sum(r["Nx"] * r["Ny"] * r["steps"] for r in stream) / wall / 1e6}, out
+
+
+def bench_serve(args):
+    import numpy as np
+    from life_amd import config as cfg
+    from life_amd import lib
+    from life_amd.serve import Server
+
+    stream = cfg.serving_stream(n=args.serve_cases, steps=args.serve_steps)
+    wl = {"workload": "serve_inverted_flag_stream", "cases": args.serve_cases, "steps_per_case": args.serve_steps,
+          "arrival_rate_per_s": 4.0, "widths": [512, 1024, 2048], "heights": [128, 256, 512],
+          "plate_length": "0.25*height", "fem_elements": [20, 60], "Re": [50, 400], "K_B": [0.1, 1.0],
+          "collision": "central_moments", "seed": 20261203, "slots": args.serve_slots,
+          "hbm_budget_bytes": args.serve_budget, "precision": "fp32_shifted populations, fp64 FEM/coupling"}
+    srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
+    print("bench: serving %d cases x %d steps" % (args.serve_cases, args.serve_steps), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ids = srv.run(stream)
+    wall = time.perf_counter() - t0
+    full, _ = serve_summary(srv, ids, wall)
+    srv.close()
+    print("bench: serving done in %.1f s: %s" % (wall, json.dumps(full)), file=sys.stderr, flush=True)
+
+    base = None
+    if args.baseline_cases > 0:
+        pre = stream[:args.baseline_cases]
+        srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
+        t0 = time.perf_counter()
+        pids = srv.run(pre)
+        pwall = time.perf_counter() - t0
+        served, pinfo = serve_summary(srv, pids, pwall)
+        srv.close()
+        print("bench: prefix served in %.1f s" % pwall, file=sys.stderr, flush=True)
+        plain, pout = run_one_at_a_time(pre, lib.LIFE_FP64)
+        tip_err = max(float(np.linalg.norm(np.array(a["tip"][:2]) - b["tip"]) / b["L"])
+                      for a, b in zip(pinfo, pout))
+        f_err = max(float(np.linalg.norm(np.array(a["total_force"][:2]) - b["force"]) /
+                          max(np.linalg.norm(b["force"]), 1e-300)) for a, b in zip(pinfo, pout))
+        base = {"prefix_cases": len(pre), "served_fp32": served, "plain_fp64_one_at_a_time": plain,
+                "throughput_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
+                "p99_ratio": plain["p99_s"] / served["p99_s"],
+                "accuracy_fp32_vs_fp64": {"max_tip_error_over_L": tip_err, "max_force_rel_error": f_err}}
+    out = {
+        "metric": "FSI cases/hour (serving engine, continuous batching)",
+        "value": full["cases_per_hour"],
+        "unit": "cases/hour",
+        "n_gpus": 1,
+        "steps": args.serve_steps,
+        "warmup": 0,
+        "ms_per_step": full["ms_per_engine_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": wl,
+        "serving": full,
+        "vs_plain_fp64_one_at_a_time": base,
+    }
+    print(json.dumps(out))
+
+
 def main():
     if "--calibrate-copy" in sys.argv:
         calibrate_copy(1 << 30, 5)
@@ -95,11 +206,20 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-cases", type=int, default=256)
+    ap.add_argument("--serve-steps", type=int, default=5000)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-budget", type=float, default=200e9)
+    ap.add_argument("--baseline-cases", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs each case on a single MI355X "
               "(multi-GPU sharding is a design note, not an implementation).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "serve":
+        bench_serve(args)
+        return
 
     import numpy as np
 

@@ -156,6 +156,67 @@ int life_case_coupling_info(const life_case *c, double out[3]);
 /* ObjectsClass::writeTotalForces quantities (Objects.cpp:517-534): Fx, Fy, Cx, Cy */
 int life_case_total_force(life_case *c, double out[4]);
 
+/* ---------------------------------------------------------------- serving
+ *
+ * Continuous batching of independent cases on one device.  The reference
+ * runs one case per process (main.cpp:33-95); it has no counterpart for
+ * this layer, which drives many layer-2 cases at once.  Requests queue in
+ * arrival order and are admitted while a slot is free and their device
+ * footprint fits the HBM budget; every engine step advances each admitted
+ * case by one timestep (lattice phase of all cases as one hipGraph launch
+ * over a packed (case, block) index, batched coupling kernels, host FP64
+ * FEM on host threads), and a finished case frees its slot for the next
+ * request.  Each case's results are those of life_case_step on the same
+ * inputs, whatever else runs beside it. */
+typedef struct life_server life_server;
+
+typedef struct life_server_config {
+    int device;
+    int precision;      /* LIFE_FP32_SHIFTED or LIFE_FP64, for every case */
+    int max_cases;      /* admission slots */
+    double hbm_budget;  /* bytes of device memory the admitted cases may hold */
+    int use_graph;      /* 1: lattice phase as one hipGraph launch per step */
+} life_server_config;
+
+enum { LIFE_REQ_QUEUED = 0, LIFE_REQ_RUNNING = 1, LIFE_REQ_DONE = 2, LIFE_REQ_FAILED = 3 };
+
+typedef struct life_request_info {
+    int status;             /* LIFE_REQ_* */
+    int steps_done, nsteps;
+    int subits;             /* subiterations of the last step */
+    int markers;            /* Lagrangian markers of the case (after admission) */
+    double t_arrival;       /* seconds since the server was created */
+    double t_admit, t_finish;
+    double sites;           /* Nx * Ny */
+    double bytes;           /* device footprint charged to the budget */
+    double total_force[4];  /* Fx Fy Cx Cy after the last step, as life_case_total_force */
+    double tip[4];          /* last marker after the last step: x y vx vy */
+} life_request_info;
+
+int life_server_create(const life_server_config *cfg, life_server **out);
+int life_server_destroy(life_server *s);
+/* Queue a request arriving now; keep_fields = 1 keeps rho, u and the marker
+ * positions of the last step for life_server_fields. */
+int life_server_submit(life_server *s, const life_params *p, const char *geometry, int nsteps, int keep_fields,
+                       int *id);
+/* One engine step: admit, advance every running case, retire finished ones. */
+int life_server_step(life_server *s, int *running, int *queued);
+/* Persistent request loop over a timed stream: request k arrives arrival[k]
+ * seconds after the call (ascending) and is submitted then; returns when
+ * all are finished.  ids[k] receives request k's id. */
+int life_server_run(life_server *s, int n, const life_params *p, const char *const *geometry, const int *nsteps,
+                    const double *arrival, int *ids);
+int life_server_info(const life_server *s, int id, life_request_info *out);
+/* Message of a failed request. */
+const char *life_server_request_error(const life_server *s, int id);
+/* rho[N], u[2N] (reference layout) and marker positions [2*markers] of a
+ * finished request submitted with keep_fields = 1. */
+int life_server_fields(const life_server *s, int id, double *rho, double *u, double *marker_pos);
+int life_server_release(life_server *s, int id);
+/* site updates, engine steps, busy seconds, graph captures, peak running
+ * cases, peak admitted bytes */
+int life_server_stats(const life_server *s, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

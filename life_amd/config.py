@@ -173,3 +173,28 @@ def inverted_flag(Nx=512, Ny=256, length_frac=0.25, n_el=20, Re=200.0, kb=0.5, u
     geom = ("FILAMENT 1 0 %.17g %.17g 0.0 0.0 %.17g %.17g 180.0 FLEXIBLE %d CLAMPED %.17g %.17g\n"
             % (x0, y0, L, h, n_el, density_ratio, E))
     return p, geom
+
+
+def serving_stream(n=256, rate=4.0, steps=5000, seed=20261203, widths=(512, 1024, 2048),
+                   heights=(128, 256, 512), n_el_range=(20, 60), Re_range=(50.0, 400.0),
+                   kb_range=(0.1, 1.0), central_moments=1):
+    """Stream of independent inverted-flag cases with seeded Poisson arrivals
+    (mean `rate` per second): width and height uniform over the given sets,
+    plate length 0.25 * height, element count uniform in n_el_range, Re
+    uniform, K_B log-uniform.  Central-moments collision (the reference's
+    more stable operator at the small relaxation times of Re = 400 on a
+    32-site plate)."""
+    rng = random.Random(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.expovariate(rate)
+        Nx = rng.choice(widths)
+        Ny = rng.choice(heights)
+        n_el = rng.randint(n_el_range[0], n_el_range[1])
+        Re = rng.uniform(Re_range[0], Re_range[1])
+        kb = math.exp(rng.uniform(math.log(kb_range[0]), math.log(kb_range[1])))
+        p, g = inverted_flag(Nx, Ny, 0.25, n_el, Re, kb, central_moments=central_moments)
+        out.append({"arrival": t, "params": p, "geometry": g, "steps": steps, "Nx": Nx, "Ny": Ny,
+                    "n_el": n_el, "Re": Re, "kb": kb})
+    return out

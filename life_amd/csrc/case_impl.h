@@ -1,0 +1,101 @@
+// Internal: the host state of one LIFE case (life_case), shared by the
+// single-case driver (coupling.cpp) and the serving engine (serve.cpp).
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fem.h"
+#include "runtime.h"
+
+namespace life {
+
+struct Body {
+    int ID;
+    int flex;  // 0 flexible, 1 rigid (eFlexibleType, defs.h:48)
+    int kind;  // 0 circle, 1 filament
+    int first = 0, n = 0;
+    std::unique_ptr<FemBody> fem;
+    // construction data kept until the markers are final
+    std::vector<double> mpos;
+    double fpos[2] = {0, 0}, fgeom[2] = {0, 0}, fangle = 0, frho = 0, fE = 0;
+    std::string fnel, fbc;
+};
+
+// Optional phase timing (environment LIFE_TIMING=1): synchronises the
+// stream at each phase boundary, so it is for diagnosis only.
+struct PhaseTimes {
+    bool on = std::getenv("LIFE_TIMING") != nullptr;
+    double t[6] = {0, 0, 0, 0, 0, 0};  // lbm, recompute, interp+d2h, fem, spread, steps
+    long subits = 0;
+    std::chrono::steady_clock::time_point last;
+    void start(life_grid* g) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        last = std::chrono::steady_clock::now();
+    }
+    void lap(life_grid* g, int k) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(g->stream));
+        const auto now = std::chrono::steady_clock::now();
+        t[k] += std::chrono::duration<double, std::milli>(now - last).count();
+        last = now;
+    }
+    void report() const {
+        if (!on || t[5] == 0) return;
+        std::fprintf(stderr, "life timing over %.0f steps (%ld subiterations), ms/step: lbm %.3f recompute %.3f "
+                     "interp %.3f fem %.3f spread %.3f\n", t[5], subits, t[0] / t[5], t[1] / t[5], t[2] / t[5],
+                     t[3] / t[5], t[4] / t[5]);
+    }
+};
+
+}  // namespace life
+
+struct life_case {
+    life_params p{};
+    life_grid* g = nullptr;
+    life::CaseScalars cs{};
+    int t = 0;
+    std::vector<life::Body> bodies;
+    std::vector<life::Body*> flex_bodies;  // flexible bodies in body order
+    int nm = 0;
+    std::vector<double> pos, vel, force, eps, ds;
+    std::vector<int> mbody;
+    bool hasIBM = false, hasFlex = false;
+    int subIt = 0, simDOFs = 0;
+    double subRes = 0.0, subNum = 0.0, subDen = 0.0, relax = 1.0;
+    std::vector<double> A;
+    std::vector<double> luw;
+    std::vector<int> luiw;
+    std::vector<double> rhs, sol;
+    bool forces_on_host = false;
+    std::unique_ptr<life::PhaseTimes> timing{new life::PhaseTimes()};
+    std::vector<hipEvent_t> kev;  // per-step stream-collide events of life_case_step_timed
+
+    life::MarkerView view(const life::Body& b) {
+        return life::MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};
+    }
+};
+
+namespace life {
+
+constexpr int kMaxSubIt = 20;  // Objects.cpp:40 (MAXIT)
+
+// Phases of ObjectsClass::objectKernel (Objects.cpp:26-60) that the
+// serving engine interleaves across cases:
+// host part of recomputeObjectVals (predictor / Aitken relaxation, new
+// marker positions and velocities in c->pos, c->vel)
+void case_recompute_host(life_case* c);
+// femKernel with the forces and epsilon in c->force, c->eps; updates
+// subRes/subNum/subDen and increments subIt
+void case_fem(life_case* c);
+// another subiteration is due (Objects.cpp:59)
+inline bool case_subiterate(const life_case* c) { return c->subIt < kMaxSubIt && c->subRes > c->p.subTol; }
+// computeEpsilon for markers [first, first+n) by the dense path
+void case_solve_eps_dense(life_case* c, int first, int n);
+
+}  // namespace life

@@ -18,80 +18,10 @@
 #include <string>
 #include <vector>
 
-#include "fem.h"
+#include "case_impl.h"
 #include "lu.h"
-#include "runtime.h"
 
 using namespace life;
-
-namespace {
-
-struct Body {
-    int ID;
-    int flex;  // 0 flexible, 1 rigid (eFlexibleType, defs.h:48)
-    int kind;  // 0 circle, 1 filament
-    int first = 0, n = 0;
-    std::unique_ptr<FemBody> fem;
-    // construction data kept until the markers are final
-    std::vector<double> mpos;
-    double fpos[2] = {0, 0}, fgeom[2] = {0, 0}, fangle = 0, frho = 0, fE = 0;
-    std::string fnel, fbc;
-};
-
-// Optional phase timing (environment LIFE_TIMING=1): synchronises the
-// stream at each phase boundary, so it is for diagnosis only.
-struct PhaseTimes {
-    bool on = std::getenv("LIFE_TIMING") != nullptr;
-    double t[6] = {0, 0, 0, 0, 0, 0};  // lbm, recompute, interp+d2h, fem, spread, steps
-    long subits = 0;
-    std::chrono::steady_clock::time_point last;
-    void start(life_grid* g) {
-        if (!on) return;
-        LIFE_HIP(hipStreamSynchronize(g->stream));
-        last = std::chrono::steady_clock::now();
-    }
-    void lap(life_grid* g, int k) {
-        if (!on) return;
-        LIFE_HIP(hipStreamSynchronize(g->stream));
-        const auto now = std::chrono::steady_clock::now();
-        t[k] += std::chrono::duration<double, std::milli>(now - last).count();
-        last = now;
-    }
-    void report() const {
-        if (!on || t[5] == 0) return;
-        std::fprintf(stderr, "life timing over %.0f steps (%ld subiterations), ms/step: lbm %.3f recompute %.3f "
-                     "interp %.3f fem %.3f spread %.3f\n", t[5], subits, t[0] / t[5], t[1] / t[5], t[2] / t[5],
-                     t[3] / t[5], t[4] / t[5]);
-    }
-};
-
-}  // namespace
-
-struct life_case {
-    life_params p{};
-    life_grid* g = nullptr;
-    CaseScalars cs{};
-    int t = 0;
-    std::vector<Body> bodies;
-    std::vector<Body*> flex_bodies;  // flexible bodies in body order
-    int nm = 0;
-    std::vector<double> pos, vel, force, eps, ds;
-    std::vector<int> mbody;
-    bool hasIBM = false, hasFlex = false;
-    int subIt = 0, simDOFs = 0;
-    double subRes = 0.0, subNum = 0.0, subDen = 0.0, relax = 1.0;
-    std::vector<double> A;
-    std::vector<double> luw;
-    std::vector<int> luiw;
-    std::vector<double> rhs, sol;
-    bool forces_on_host = false;
-    std::unique_ptr<PhaseTimes> timing{new PhaseTimes()};
-    std::vector<hipEvent_t> kev;  // per-step stream-collide events of life_case_step_timed
-
-    MarkerView view(const Body& b) {
-        return MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};
-    }
-};
 
 namespace {
 
@@ -239,16 +169,7 @@ void finalise_markers(life_case* c) {
 // Later calls (flexible bodies only): banded systems assembled and solved
 // on the device, with the dense host path for any system that is not banded.
 void compute_epsilon(life_case* c, bool initial) {
-    auto solve = [&](int first, int n) {
-        c->A.resize((size_t)n * n);
-        ibm_epsilon_matrix(c->g, first, n, c->A.data());
-        c->rhs.assign(n, 1.0);
-        c->sol.assign(n, 0.0);
-        lu_solve_sparse(n, 0, c->A.data(), c->rhs.data(), c->sol.data(), c->luw, c->luiw);
-        std::memcpy(&c->eps[first], c->sol.data(), sizeof(double) * n);
-        LIFE_HIP(hipMemcpyAsync(c->g->md.eps + first, &c->eps[first], sizeof(double) * n, hipMemcpyHostToDevice,
-                                c->g->stream));
-    };
+    auto solve = [&](int first, int n) { case_solve_eps_dense(c, first, n); };
     if (c->p.uni_epsilon) {
         solve(0, c->nm);
     } else if (initial) {
@@ -277,8 +198,10 @@ void for_flexible(life_case* c, Fn&& fn) {
     for (int q = 0; q < nf; q++) fn(*c->flex_bodies[q]);
 }
 
-// ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
-void recompute_object_vals(life_case* c) {
+}  // namespace
+
+// Host part of ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
+void life::case_recompute_host(life_case* c) {
     if (c->subIt == 0) {
         for_flexible(c, [&](Body& b) {
             b.fem->reset_values();
@@ -294,6 +217,40 @@ void recompute_object_vals(life_case* c) {
         const double w = c->relax;
         for_flexible(c, [&](Body& b) { b.fem->relax(w, c->view(b)); });
     }
+}
+
+// ObjectsClass::femKernel, Objects.cpp:63-99 (ORDERED: sums in body order).
+void life::case_fem(life_case* c) {
+    for_flexible(c, [&](Body& b) { b.fem->dynamic(c->view(b)); });
+    double res = 0.0, num = 0.0, den = 0.0;
+    for (Body* b : c->flex_bodies) {
+        res += b->fem->sub_res();
+        num += b->fem->sub_num();
+        den += b->fem->sub_den();
+    }
+    c->subRes = std::sqrt(res) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
+    c->subNum = num;
+    c->subDen = den;
+    c->subIt++;
+}
+
+// computeEpsilon's solve of one system by the dense path.
+void life::case_solve_eps_dense(life_case* c, int first, int n) {
+    c->A.resize((size_t)n * n);
+    ibm_epsilon_matrix(c->g, first, n, c->A.data());
+    c->rhs.assign(n, 1.0);
+    c->sol.assign(n, 0.0);
+    lu_solve_sparse(n, 0, c->A.data(), c->rhs.data(), c->sol.data(), c->luw, c->luiw);
+    std::memcpy(&c->eps[first], c->sol.data(), sizeof(double) * n);
+    LIFE_HIP(hipMemcpyAsync(c->g->md.eps + first, &c->eps[first], sizeof(double) * n, hipMemcpyHostToDevice,
+                            c->g->stream));
+}
+
+namespace {
+
+// ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
+void recompute_object_vals(life_case* c) {
+    case_recompute_host(c);
     life_grid* g = c->g;
     LIFE_HIP(hipMemcpyAsync(g->md.pos, c->pos.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
     LIFE_HIP(hipMemcpyAsync(g->md.vel, c->vel.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
@@ -303,7 +260,6 @@ void recompute_object_vals(life_case* c) {
 
 // ObjectsClass::objectKernel, Objects.cpp:26-60.
 void object_kernel(life_case* c) {
-    const int MAXIT = 20;
     c->subIt = 0;
     c->forces_on_host = false;
     PhaseTimes& T = *c->timing;
@@ -315,20 +271,9 @@ void object_kernel(life_case* c) {
         download_forces(c);
         T.lap(c->g, 2);
         T.subits++;
-        // femKernel, Objects.cpp:63-99 (ORDERED: sums in body order)
-        for_flexible(c, [&](Body& b) { b.fem->dynamic(c->view(b)); });
+        case_fem(c);
         T.lap(c->g, 3);
-        double res = 0.0, num = 0.0, den = 0.0;
-        for (Body* b : c->flex_bodies) {
-            res += b->fem->sub_res();
-            num += b->fem->sub_num();
-            den += b->fem->sub_den();
-        }
-        c->subRes = std::sqrt(res) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
-        c->subNum = num;
-        c->subDen = den;
-        c->subIt++;
-    } while (c->subIt < MAXIT && c->subRes > c->p.subTol);
+    } while (case_subiterate(c));
     ibm_spread(c->g);
     T.lap(c->g, 4);
     if (c->hasFlex) {
@@ -341,7 +286,7 @@ void object_kernel(life_case* c) {
                                 c->g->stream));
         c->forces_on_host = true;
     }
-    if (c->subIt == MAXIT) throw std::runtime_error("Subiteration scheme hit 20 iterations");
+    if (c->subIt == kMaxSubIt) throw std::runtime_error("Subiteration scheme hit 20 iterations");
 }
 
 void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end) {

@@ -30,20 +30,26 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
 // (they only change row, a multiple of Ny elements).
 constexpr int kPushBlock = 256;
 
+// Pointer to T in the global address space (for pointers read from device
+// tables, which the compiler cannot otherwise place).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
 template <typename T>
 struct PushStage {
     T slot[6][kPushBlock + 2];
 };
 
-template <typename T, bool NT>
-__device__ __forceinline__ void put(T* p, T v) {
+// P: T* or a pointer to T in the global address space.
+template <typename T, bool NT, typename P>
+__device__ __forceinline__ void put(P p, T v) {
     if (NT) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
 // fo: the nine post-collision values of site id = base + t (row i, column j).
-template <typename T, bool NT>
-__device__ __forceinline__ void push_store_realigned(T* __restrict__ dst, long long S, int Nx, int Ny, long long N,
+template <typename T, bool NT, typename DP>
+__device__ __forceinline__ void push_store_realigned(DP __restrict__ dst, long long S, int Nx, int Ny, long long N,
                                                      long long base, int t, long long id, int i, int j,
                                                      const T fo[9], bool valid, PushStage<T>& st) {
     const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;

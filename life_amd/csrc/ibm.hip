@@ -14,9 +14,7 @@ namespace life {
 // IBMNodeClass::findSupport (IBMNode.cpp:139-179): lattice sites within 1.5
 // spacings of the marker in both directions, scanned i-major over the 5x5
 // block around the nearest site, kept in scan order (at most 9).
-__global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m.n) return;
+__device__ inline void find_support_one(const MarkerDev& m, const GridScalars& g, int flexible_only, int k) {
     if (flexible_only && m.body_flex[m.body[k]] != 0) return;
     const double Dx = g.Dx;
     const double x = m.pos[2 * k], y = m.pos[2 * k + 1];
@@ -48,13 +46,24 @@ __global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
     m.supp_cnt[k] = n;
 }
 
+__global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m.n) find_support_one(m, g, flexible_only, k);
+}
+
+__global__ void k_find_support_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list,
+                                     int flexible_only) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < c.m.n) find_support_one(c.m, c.g, flexible_only, k);
+}
+
 // IBMNodeClass::computeDs (IBMNode.cpp:182-204): distance (lattice units)
 // to the nearest other marker of the same body, starting from 10.  One
 // workgroup per body; the body's positions are staged in LDS when they fit.
 constexpr int kDsLdsMarkers = 2048;
-__global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
-    extern __shared__ double spos[];
-    const int b = blockIdx.x;
+__device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g, int flexible_only, int b,
+                                       double* spos) {
     if (flexible_only && m.body_flex[b] != 0) return;
     const int first = m.body_first[b], cnt = m.body_n[b];
     const bool lds = cnt <= kDsLdsMarkers;
@@ -76,6 +85,17 @@ __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
         const double mag = sqrt(d2min) / g.Dx;
         m.ds[first + k] = mag < 10.0 ? mag : 10.0;
     }
+}
+
+__global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
+    extern __shared__ double spos[];
+    compute_ds_body(m, g, flexible_only, blockIdx.x, spos);
+}
+
+__global__ void k_compute_ds_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list, int flexible_only) {
+    extern __shared__ double spos[];
+    const CaseDev& c = cases[list[blockIdx.y]];
+    if ((int)blockIdx.x < c.nbodies) compute_ds_body(c.m, c.g, flexible_only, blockIdx.x, spos);
 }
 
 // A_ij of ObjectsClass::computeEpsilon (Objects.cpp:271-300) for the system
@@ -107,13 +127,9 @@ __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, d
 // dense path instead.
 
 // One thread per band entry; grid.y = system.
-__global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
-                                   const int* __restrict__ sys_n, int KB, double* __restrict__ band,
-                                   const long long* __restrict__ band_off) {
-    const int sys = blockIdx.y;
-    const int first = sys_first[sys], n = sys_n[sys];
+__device__ inline void eps_band_entry(const MarkerDev& m, const GridScalars& g, int first, int n, int KB,
+                                      double* __restrict__ band, long long t) {
     const int W = 3 * KB + 1;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * W) return;
     const int r = (int)(t / W), q = (int)(t - (long long)r * W);
     const int c = r - KB + q;
@@ -130,24 +146,51 @@ __global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __rest
         }
         a = a * m.ds[mj];
     }
-    band[band_off[sys] + t] = a;
+    band[t] = a;
+}
+
+__global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                   const int* __restrict__ sys_n, int KB, double* __restrict__ band,
+                                   const long long* __restrict__ band_off) {
+    const int sys = blockIdx.y;
+    eps_band_entry(m, g, sys_first[sys], sys_n[sys], KB, band + band_off[sys],
+                   (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Pairs outside the band that could couple (support of i within 1.5 of
 // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings).
 // One thread per pair r < c - KB (the test is symmetric); grid.y = system.
-__global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
-                                const int* __restrict__ sys_n, int KB, int* __restrict__ outside) {
-    const int sys = blockIdx.y;
-    const int first = sys_first[sys], n = sys_n[sys];
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void eps_far_pair(const MarkerDev& m, const GridScalars& g, int first, int n, int KB,
+                                    int* __restrict__ outside, long long t) {
     if (t >= (long long)n * n) return;
     const int r = (int)(t / n), c = (int)(t - (long long)r * n);
     if (c <= r + KB) return;
     const double lim = 3.01 * g.Dx;
     const int mi = first + r, mj = first + c;
     if (fabs(m.pos[2 * mi] - m.pos[2 * mj]) < lim && fabs(m.pos[2 * mi + 1] - m.pos[2 * mj + 1]) < lim)
-        outside[sys] = 1;
+        *outside = 1;
+}
+
+__global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                const int* __restrict__ sys_n, int KB, int* __restrict__ outside) {
+    const int sys = blockIdx.y;
+    eps_far_pair(m, g, sys_first[sys], sys_n[sys], KB, outside + sys,
+                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched assembly: grid.y = entry of the system table.
+__global__ void k_eps_band_entries_batch(const CaseDev* __restrict__ cases, const CaseSys* __restrict__ sys, int KB) {
+    const CaseSys q = sys[blockIdx.y];
+    const CaseDev& c = cases[q.c];
+    eps_band_entry(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], KB, c.eps.band + c.eps.off[q.k],
+                   (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_eps_far_check_batch(const CaseDev* __restrict__ cases, const CaseSys* __restrict__ sys, int KB) {
+    const CaseSys q = sys[blockIdx.y];
+    const CaseDev& c = cases[q.c];
+    eps_far_pair(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], KB, c.eps.outside + q.k,
+                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Canonical elimination (partial pivoting, first maximum) of one banded
@@ -155,64 +198,88 @@ __global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restric
 //
 // With lower and upper bandwidth KB, every non-zero that step k of the
 // dense elimination reads or writes lies in rows k..k+KB and columns
-// k..k+2KB (row swaps widen the upper band by KB).  Lane c of the wavefront
-// keeps column k+c of that window in registers (w[r] = a(k+r, k+c)); each
-// step broadcasts column k, picks the pivot, swaps, forms the multipliers,
-// updates the other columns (each a_ij - l_i a_kj exactly once from the same
-// operands as the dense algorithm), emits row k of U, and slides the window
-// one row and one column.  Back substitution then runs on one lane in
-// increasing column order.  The skipped dense operations all involve an
-// exact zero, so the values equal the dense elimination's.  The band is
-// staged in LDS (rows of U replace rows of the band as they are emitted)
-// when it fits in lds_rows rows.
-template <int KB>
-__global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
-                                                       const int* __restrict__ outside, double* __restrict__ band,
-                                                       const long long* __restrict__ band_off,
-                                                       double* __restrict__ ywork, double* __restrict__ eps,
-                                                       int lds_rows) {
+// k..k+2KB (row swaps widen the upper band by KB).  That window stays in
+// registers: register slot r holds row k+r, and lane L < C holds the
+// window column j with j = L (mod C), so columns never move between lanes.
+// Step k:
+//  - column k is broadcast from its lane (v_readlane, no LDS traffic);
+//  - the pivot is the first row of maximum magnitude: the maximum over the
+//    window (rows past n hold zeros) and then the first row equal to it,
+//    which is the sequential first-maximum search for non-NaN entries;
+//  - rows k and k+p are swapped, and lane r forms the multiplier
+//    l_r = a(k+r, k) / a(k, k) (one division for the whole wavefront);
+//  - every other column gets a_ij - l_i a_kj exactly once, from the same
+//    operands as the dense algorithm, and row k of U is emitted.
+// The slots then shift by one row (row k+R enters), and column k's lane
+// takes column k+C, whose entries in rows k+1..k+KB are structural zeros.
+// Back substitution runs on one lane in increasing column order.  The
+// skipped dense operations all involve an exact zero, so the values equal
+// the dense elimination's.  The band is staged in LDS (rows of U replace
+// rows of the band as they are emitted) when it fits in lds_rows rows.
+
+// Lane `lane`'s value, as a per-lane (VGPR) copy: the identity DPP move
+// keeps the pivot logic in vector registers.
+__device__ inline double bcast64(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    const unsigned vlo = (unsigned)__builtin_amdgcn_update_dpp(0, lo, 0xE4, 0xF, 0xF, false);
+    const unsigned vhi = (unsigned)__builtin_amdgcn_update_dpp(0, hi, 0xE4, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)vhi << 32) | vlo);
+}
+
+// Lane `lane`'s value as a wavefront-uniform (SGPR) value.
+__device__ inline double uniform64(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// B: the band (rows of U replace it), y: the right-hand side; both either
+// in LDS or in global memory (LDS_ selects the instantiation, so that every
+// access is compiled for its own address space).
+template <int KB, bool LDS_>
+__device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __restrict__ B, double* __restrict__ y,
+                                           double* __restrict__ eps) {
     constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
-    extern __shared__ double sm[];
-    const int sys = blockIdx.x;
-    if (outside[sys]) return;
-    const int first = sys_first[sys], n = sys_n[sys];
-    const bool in_lds = n <= lds_rows;
-    double* G = band + band_off[sys];
-    double* B = in_lds ? sm : G;
-    double* y = in_lds ? sm + (long long)n * W : ywork + first;
-    const int c = threadIdx.x;
-    if (in_lds) {
-        for (int t = c; t < n * W; t += 64) B[t] = G[t];
-        __syncthreads();
-    }
+    const int L = threadIdx.x;
     auto A0 = [&](int r, int col) -> double {
-        if (c >= C || r >= n || col < 0 || col >= n) return 0.0;
+        if (L >= C || r >= n || col < 0 || col >= n) return 0.0;
         const int q = col - r + KB;
         return (q >= 0 && q < W) ? B[r * W + q] : 0.0;
     };
     double w[R], yw[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        w[r] = A0(r, c);
+        w[r] = A0(r, L);
         yw[r] = r < n ? 1.0 : 0.0;
     }
+    int pl = 0;   // lane holding column k
+    int off = L;  // this lane holds column k + off
     for (int k = 0; k < n; k++) {
-        const double incoming = A0(k + R, k + 1 + c);  // row entering the window next
+        const double incoming = A0(k + R, L == pl ? k + C : k + off);  // row k+R, after the step
         const int rmax = min(KB, n - 1 - k);
-        double col[R];
+        double col[R], a[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) col[r] = __shfl(w[r], 0);
+        for (int r = 0; r < R; r++) {
+            col[r] = bcast64(w[r], pl);
+            a[r] = fabs(col[r]);
+        }
+        // maximum magnitude (tree), then the first row that attains it
+        double mx[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) mx[r] = a[r];
+#pragma unroll
+        for (int h = 1; h < R; h *= 2)
+#pragma unroll
+            for (int r = 0; r + h < R; r += 2 * h) mx[r] = mx[r + h] > mx[r] ? mx[r + h] : mx[r];
+        const double best = mx[0];
         int p = 0;
-        double best = fabs(col[0]);
 #pragma unroll
-        for (int r = 1; r < R; r++)
-            if (r <= rmax) {
-                const double a = fabs(col[r]);
-                if (a > best) {
-                    best = a;
-                    p = r;
-                }
-            }
+        for (int r = R - 1; r > 0; r--)
+            if (a[r] == best) p = r;
+        if (a[0] == best) p = 0;
         if (best != 0.0) {
             if (p != 0) {
                 const double tw = w[0], tc = col[0], ty = yw[0];
@@ -228,29 +295,35 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
                     }
             }
             const double piv = col[0];
+            double num = col[1];
+#pragma unroll
+            for (int r = 2; r < R; r++) num = L == r ? col[r] : num;
+            const double lv = num / piv;  // lane r: l_r
 #pragma unroll
             for (int r = 1; r < R; r++)
                 if (r <= rmax) {
-                    const double l = col[r] / piv;
-                    if (c == 0) w[r] = l;
-                    else w[r] = w[r] - l * w[0];
+                    const double l = uniform64(lv, r);
+                    const double upd = w[r] - l * w[0];
+                    w[r] = L == pl ? 0.0 : upd;
                     yw[r] = yw[r] - l * yw[0];
                 }
         }
         // row k of U (in place of row k of the band) and the final y_k
-        if (c < C && k + c < n) B[k * W + (c + KB)] = w[0];
-        if (c == 0) y[k] = yw[0];
-        // slide: drop row k and column k, take in row k+R
+        if (L < C && k + off < n) B[k * W + (off + KB)] = w[0];
+        if (L == 0) y[k] = yw[0];
+        // drop row k, take in row k+R; column k's lane takes column k+C
 #pragma unroll
         for (int r = 0; r < R - 1; r++) {
-            w[r] = __shfl_down(w[r + 1], 1);
+            w[r] = w[r + 1];
             yw[r] = yw[r + 1];
         }
         w[R - 1] = incoming;
         yw[R - 1] = k + R < n ? 1.0 : 0.0;
+        off = L == pl ? C - 1 : off - 1;
+        pl = pl == C - 1 ? 0 : pl + 1;
     }
     __syncthreads();
-    if (c == 0) {
+    if (L == 0) {
         double xw[2 * KB];
 #pragma unroll
         for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
@@ -269,13 +342,49 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
     }
 }
 
+template <int KB>
+__device__ __forceinline__ void eps_band_solve_body(int first, int n, double* __restrict__ G, double* __restrict__ ywork,
+                                           double* __restrict__ eps, int lds_rows, double* sm) {
+    constexpr int W = 3 * KB + 1;
+    if (n <= lds_rows) {
+        const int nw = n * W;
+#pragma unroll 8
+        for (int t = threadIdx.x; t < nw; t += 64) sm[t] = G[t];
+        __syncthreads();
+        eps_band_solve_impl<KB, true>(first, n, sm, sm + (long long)nw, eps);
+    } else {
+        eps_band_solve_impl<KB, false>(first, n, G, ywork + first, eps);
+    }
+}
+
+template <int KB>
+__global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
+                                                       const int* __restrict__ outside, double* __restrict__ band,
+                                                       const long long* __restrict__ band_off,
+                                                       double* __restrict__ ywork, double* __restrict__ eps,
+                                                       int lds_rows) {
+    extern __shared__ double sm[];
+    const int sys = blockIdx.x;
+    if (outside[sys]) return;
+    eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm);
+}
+
+template <int KB>
+__global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __restrict__ cases,
+                                                             const CaseSys* __restrict__ sys, int lds_rows) {
+    extern __shared__ double sm[];
+    const CaseSys q = sys[blockIdx.x];
+    const CaseDev& c = cases[q.c];
+    if (c.eps.outside[q.k]) return;
+    eps_band_solve_body<KB>(c.eps.first[q.k], c.eps.n[q.k], c.eps.band + c.eps.off[q.k], c.eps.y, c.m.eps, lds_rows,
+                            sm);
+}
+
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
 // the supports (GridClass::macroscopic values: force_xy only) interpolated
 // with the support deltas, then F = 2 (Dt/Dx rho_I U - (rho u)_I).
 template <typename T>
-__global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m.n) return;
+__device__ inline void interpolate_one(const T* __restrict__ f, const MarkerDev& m, const GridScalars& g, int k) {
     double iR = 0.0, mx = 0.0, my = 0.0;
     const int cnt = m.supp_cnt[k];
     for (int s = 0; s < cnt; s++) {
@@ -293,6 +402,20 @@ __global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars 
     m.force[2 * k + 1] = 2.0 * (vs * iR * m.vel[2 * k + 1] - my);
 }
 
+template <typename T>
+__global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m.n) interpolate_one(f, m, g, k);
+}
+
+// Batched: the populations after this step's stream-collide.
+template <typename T>
+__global__ void k_interpolate_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < c.m.n) interpolate_one(static_cast<const T*>(c.dst), c.m, c.g, k);
+}
+
 // ---------------------------------------------------------------- spread
 // The reference adds marker contributions to each site in marker order
 // (ORDERED, IBMNode.cpp:67-83).  Here every (marker, support) pair takes a
@@ -300,19 +423,15 @@ __global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars 
 // scattered into per-site lists, and one thread per site sorts its list by
 // marker and sums it from zero: the same additions in the same order.
 
-__global__ void k_spread_clear(double* __restrict__ fibm, long long S, const int* __restrict__ uniq,
-                               const int* __restrict__ uniq_n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void spread_clear_one(double* __restrict__ fibm, long long S, const int* __restrict__ uniq,
+                                        const int* __restrict__ uniq_n, int k) {
     if (k >= *uniq_n) return;
     const long long id = uniq[k];
     fibm[id] = 0.0;
     fibm[S + id] = 0.0;
 }
 
-__global__ void k_spread_reset(int* uniq_n) { *uniq_n = 0; }
-
-__global__ void k_spread_count(MarkerDev m, SpreadScratch sc) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void spread_count_one(const MarkerDev& m, const SpreadScratch& sc, int p) {
     if (p >= m.n * kSupp) return;
     const int k = p / kSupp, s = p - k * kSupp;
     if (s >= m.supp_cnt[k]) return;
@@ -326,8 +445,7 @@ __global__ void k_spread_count(MarkerDev m, SpreadScratch sc) {
     }
 }
 
-__global__ void k_spread_scatter(MarkerDev m, SpreadScratch sc) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void spread_scatter_one(const MarkerDev& m, const SpreadScratch& sc, int p) {
     if (p >= m.n * kSupp) return;
     const int k = p / kSupp, s = p - k * kSupp;
     if (s >= m.supp_cnt[k]) return;
@@ -340,8 +458,8 @@ __global__ void k_spread_scatter(MarkerDev m, SpreadScratch sc) {
     sc.list[(long long)sc.site_uid[site] * kMaxPerSite + slot] = p;
 }
 
-__global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restrict__ fibm, long long S) {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void spread_reduce_one(const MarkerDev& m, const SpreadScratch& sc, double* __restrict__ fibm,
+                                         long long S, int u) {
     if (u >= *sc.uniq_n) return;
     const int site = sc.uniq[u];
     int n = sc.site_count[site];
@@ -370,6 +488,94 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
     sc.site_count[site] = 0;
 }
 
+__global__ void k_spread_clear(double* __restrict__ fibm, long long S, const int* __restrict__ uniq,
+                               const int* __restrict__ uniq_n) {
+    spread_clear_one(fibm, S, uniq, uniq_n, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_spread_reset(int* uniq_n) { *uniq_n = 0; }
+
+__global__ void k_spread_count(MarkerDev m, SpreadScratch sc) {
+    spread_count_one(m, sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_spread_scatter(MarkerDev m, SpreadScratch sc) {
+    spread_scatter_one(m, sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restrict__ fibm, long long S) {
+    spread_reduce_one(m, sc, fibm, S, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched spread, grid.y = case (the reset: one thread per case).
+__global__ void k_spread_clear_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < c.sc.cap; k += gridDim.x * blockDim.x)
+        spread_clear_one(c.fibm, c.g.S, c.sc.uniq, c.sc.uniq_n, k);
+}
+
+__global__ void k_spread_reset_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) *cases[list[q]].sc.uniq_n = 0;
+}
+
+__global__ void k_spread_count_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    spread_count_one(c.m, c.sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_spread_scatter_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    spread_scatter_one(c.m, c.sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_spread_reduce_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    spread_reduce_one(c.m, c.sc, c.fibm, c.g.S, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Packed marker staging <-> per-case marker arrays.
+__global__ void k_markers_in_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list,
+                                   const double* __restrict__ stage) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c.m.n) return;
+    const double* q = stage + 4 * (long long)(c.moff + k);
+    c.m.pos[2 * k] = q[0];
+    c.m.pos[2 * k + 1] = q[1];
+    c.m.vel[2 * k] = q[2];
+    c.m.vel[2 * k + 1] = q[3];
+}
+
+__global__ void k_markers_out_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list,
+                                    double* __restrict__ stage) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c.m.n) return;
+    double* q = stage + 3 * (long long)(c.moff + k);
+    q[0] = c.m.force[2 * k];
+    q[1] = c.m.force[2 * k + 1];
+    q[2] = c.m.eps[k];
+}
+
+__global__ void k_eps_outside_clear_batch(const CaseDev* __restrict__ cases, const CaseSys* __restrict__ sys,
+                                          int nsys) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nsys) cases[sys[q].c].eps.outside[sys[q].k] = 0;
+}
+
+__global__ void k_eps_outside_gather_batch(const CaseDev* __restrict__ cases, const CaseSys* __restrict__ sys,
+                                           int nsys, int* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nsys) out[q] = cases[sys[q].c].eps.outside[sys[q].k];
+}
+
+static int eps_lds_rows(int KB, int max_n) {
+    const int W = 3 * KB + 1;
+    const int lds_rows = (int)((60 * 1024) / (sizeof(double) * (W + 1)));
+    return max_n < lds_rows ? max_n : lds_rows;
+}
+
 void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
                      const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
                      double* ywork, hipStream_t s) {
@@ -382,8 +588,7 @@ void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_fi
                                                                                  band_off);
     k_eps_far_check<<<dim3((unsigned)((np + 255) / 256), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, outside);
     // band, then right-hand side, in LDS for systems of up to lds_rows rows
-    const int lds_rows = (int)((60 * 1024) / (sizeof(double) * (W + 1)));
-    const int rows = max_n < lds_rows ? max_n : lds_rows;
+    const int rows = eps_lds_rows(KB, max_n);
     k_eps_band_solve<8><<<nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(sys_first, sys_n, outside, band,
                                                                                    band_off, ywork, m.eps, rows);
 }
@@ -426,6 +631,67 @@ void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, Sprea
     k_spread_count<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
     k_spread_scatter<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
     k_spread_reduce<<<(cap + 127) / 128, 128, 0, s>>>(m, sc, fibm, g.S);
+}
+
+// ---------------------------------------------------------------- batched launchers
+void launch_find_support_batch(const CaseDev* cases, const int* list, int n, int max_markers, int flexible_only,
+                               hipStream_t s) {
+    if (n <= 0 || max_markers <= 0) return;
+    k_find_support_batch<<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list,
+                                                                                                 flexible_only);
+}
+
+void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int max_bodies, int flexible_only,
+                             hipStream_t s) {
+    if (n <= 0 || max_bodies <= 0) return;
+    k_compute_ds_batch<<<dim3((unsigned)max_bodies, (unsigned)n), 256, sizeof(double) * 2 * kDsLdsMarkers, s>>>(
+        cases, list, flexible_only);
+}
+
+void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
+                           hipStream_t s) {
+    constexpr int KB = 8, W = 3 * KB + 1;
+    if (nsys <= 0 || max_n <= 0) return;
+    const unsigned nb = (unsigned)((nsys + 255) / 256);
+    k_eps_outside_clear_batch<<<nb, 256, 0, s>>>(cases, sys, nsys);
+    const long long ne = (long long)max_n * W, np = (long long)max_n * max_n;
+    k_eps_band_entries_batch<<<dim3((unsigned)((ne + 255) / 256), (unsigned)nsys), 256, 0, s>>>(cases, sys, KB);
+    k_eps_far_check_batch<<<dim3((unsigned)((np + 255) / 256), (unsigned)nsys), 256, 0, s>>>(cases, sys, KB);
+    const int rows = eps_lds_rows(KB, max_n);
+    k_eps_band_solve_batch<KB><<<(unsigned)nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(cases, sys, rows);
+    k_eps_outside_gather_batch<<<nb, 256, 0, s>>>(cases, sys, nsys, outside_out);
+}
+
+template <typename T>
+void launch_interpolate_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s) {
+    if (n <= 0 || max_markers <= 0) return;
+    k_interpolate_batch<T><<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list);
+}
+template void launch_interpolate_batch<double>(const CaseDev*, const int*, int, int, hipStream_t);
+template void launch_interpolate_batch<float>(const CaseDev*, const int*, int, int, hipStream_t);
+
+void launch_spread_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s) {
+    if (n <= 0) return;
+    const int np = max_markers * kSupp;
+    const unsigned nc = (unsigned)((np + 255) / 256);
+    if (np > 0) k_spread_clear_batch<<<dim3(nc, (unsigned)n), 256, 0, s>>>(cases, list);
+    k_spread_reset_batch<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cases, list, n);
+    if (np <= 0) return;
+    k_spread_count_batch<<<dim3(nc, (unsigned)n), 256, 0, s>>>(cases, list);
+    k_spread_scatter_batch<<<dim3(nc, (unsigned)n), 256, 0, s>>>(cases, list);
+    k_spread_reduce_batch<<<dim3((unsigned)((np + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list);
+}
+
+void launch_markers_in_batch(const CaseDev* cases, const int* list, int n, int max_markers, const double* stage,
+                             hipStream_t s) {
+    if (n <= 0 || max_markers <= 0) return;
+    k_markers_in_batch<<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list, stage);
+}
+
+void launch_markers_out_batch(const CaseDev* cases, const int* list, int n, int max_markers, double* stage,
+                              hipStream_t s) {
+    if (n <= 0 || max_markers <= 0) return;
+    k_markers_out_batch<<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list, stage);
 }
 
 }  // namespace life

@@ -95,4 +95,76 @@ void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hi
 void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,
                    hipStream_t s);
 
+// ---------------------------------------------------------------- serving
+// Batched forms of the kernels above for the serving engine.  Every
+// admitted case has one CaseDev entry in a device table that the host
+// rewrites before each step; a launch covers the cases named by `list`
+// (indices into the table; grid.y = list entry) or, for the stream-collide,
+// a block table of (case, block) pairs.  Each case's values are computed
+// exactly as the single-case kernels compute them.
+
+// Banded epsilon systems of one case (device arrays of its life_grid).
+struct EpsDev {
+    int nsys;
+    const int* first;
+    const int* n;
+    const long long* off;
+    int* outside;
+    double* band;
+    double* y;
+};
+
+struct CaseDev {
+    GridScalars g;
+    const void* src;   // populations at the start of the step
+    void* dst;         // populations after it (the current ones during the IBM phase)
+    const double* rho0;
+    const double* u0;
+    double* fibm;
+    int explicit_state;
+    int convective;
+    BCArgs bc;
+    double* ucol;
+    MarkerDev m;
+    SpreadScratch sc;
+    int nbodies;
+    int moff;          // first marker of the case in the packed staging arrays
+    EpsDev eps;
+};
+
+struct CaseBlock {
+    int c;    // table index
+    int blk;  // 256-site block of that case
+};
+
+struct CaseSys {
+    int c;  // table index
+    int k;  // system of that case
+};
+
+// stream-collide over a block table (all cases of one collision operator)
+void launch_push32_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s);
+void launch_push64_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s);
+void launch_support_push32_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s);
+void launch_support_push64_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s);
+template <typename T>
+void launch_bc_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_nbc, hipStream_t s);
+template <typename T>
+void launch_convective_batch(const CaseDev* cases, const int* list, int n, bool ordered, hipStream_t s);
+// packed marker staging: in = [x, y, vx, vy] per marker, out = [Fx, Fy, eps]
+void launch_markers_in_batch(const CaseDev* cases, const int* list, int n, int max_markers, const double* stage,
+                             hipStream_t s);
+void launch_markers_out_batch(const CaseDev* cases, const int* list, int n, int max_markers, double* stage,
+                              hipStream_t s);
+void launch_find_support_batch(const CaseDev* cases, const int* list, int n, int max_markers, int flexible_only,
+                               hipStream_t s);
+void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int max_bodies, int flexible_only,
+                             hipStream_t s);
+// outside_out[q] = 1 for a listed system that is not banded (no solve done)
+void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
+                           hipStream_t s);
+template <typename T>
+void launch_interpolate_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);
+void launch_spread_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);
+
 }  // namespace life

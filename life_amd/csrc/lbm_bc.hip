@@ -67,11 +67,9 @@ __device__ void regularised64(const GridScalars& g, int type, int nx, int ny, in
 // dst.  rho_n/u_n start as the site's end-of-previous-step moments (the
 // reference swaps its rho/u arrays at the start of the step, Grid.cpp:43-45).
 template <typename T, bool CM>
-__global__ __launch_bounds__(256) void k_bc(const T* __restrict__ src, T* __restrict__ dst, GridScalars g,
-                                            BCArgs a, const double* __restrict__ fibm,
-                                            const double* __restrict__ rho0, const double* __restrict__ u0,
-                                            int explicit_state) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, const GridScalars& g, const BCArgs& a,
+                               const double* __restrict__ fibm, const double* __restrict__ rho0,
+                               const double* __restrict__ u0, int explicit_state, int k) {
     if (k >= a.nbc) return;
     const BCSite b = a.sites[k];
     const long long S = g.S, id = b.id;
@@ -144,6 +142,21 @@ __global__ __launch_bounds__(256) void k_bc(const T* __restrict__ src, T* __rest
     for (int v = 0; v < kNV; v++) Pops<T>::put(dst, S, v, id, fs[v]);
 }
 
+template <typename T, bool CM>
+__global__ __launch_bounds__(256) void k_bc(const T* __restrict__ src, T* __restrict__ dst, GridScalars g,
+                                            BCArgs a, const double* __restrict__ fibm,
+                                            const double* __restrict__ rho0, const double* __restrict__ u0,
+                                            int explicit_state) {
+    bc_site<T, CM>(src, dst, g, a, fibm, rho0, u0, explicit_state, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T, bool CM>
+__global__ __launch_bounds__(256) void k_bc_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    bc_site<T, CM>(static_cast<const T*>(c.src), static_cast<T*>(c.dst), c.g, c.bc, c.fibm, c.rho0, c.u0,
+                   c.explicit_state, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 template <typename T>
 void launch_bc(const T* src, T* dst, const GridScalars& g, const BCArgs& a, const double* fibm, const double* rho0,
                const double* u0, bool explicit_state, hipStream_t s) {
@@ -157,16 +170,15 @@ void launch_bc(const T* src, T* dst, const GridScalars& g, const BCArgs& a, cons
 // velocity of the last three columns.  One workgroup.  ordered = true keeps
 // the reference's serial left-to-right mean (plain method); otherwise an
 // FP64 tree reduction is used.
+constexpr int kConvThreads = 1024;
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_convective_speed(const T* __restrict__ src, GridScalars g,
-                                                           const double* __restrict__ fibm,
-                                                           const double* __restrict__ u0, int explicit_state,
-                                                           double* __restrict__ ucol, double* __restrict__ delU,
-                                                           int ordered) {
+__device__ inline void convective_speed_block(const T* __restrict__ src, const GridScalars& g,
+                                              const double* __restrict__ fibm, const double* __restrict__ u0,
+                                              int explicit_state, double* __restrict__ ucol,
+                                              double* __restrict__ delU, int ordered, double* red, double& uOutS) {
     const int Ny = g.Ny, Nx = g.Nx;
     const long long S = g.S;
-    __shared__ double red[1024];
-    __shared__ double uOutS;
     double part = 0.0;
     for (int j = threadIdx.x; j < Ny; j += blockDim.x) {
         for (int c = 0; c < 3; c++) {
@@ -214,9 +226,47 @@ __global__ __launch_bounds__(1024) void k_convective_speed(const T* __restrict__
 }
 
 template <typename T>
+__global__ __launch_bounds__(kConvThreads) void k_convective_speed(const T* __restrict__ src, GridScalars g,
+                                                                   const double* __restrict__ fibm,
+                                                                   const double* __restrict__ u0, int explicit_state,
+                                                                   double* __restrict__ ucol,
+                                                                   double* __restrict__ delU, int ordered) {
+    __shared__ double red[kConvThreads];
+    __shared__ double uOutS;
+    convective_speed_block(src, g, fibm, u0, explicit_state, ucol, delU, ordered, red, uOutS);
+}
+
+// Serving: one workgroup per listed case (cases with a convective outlet).
+template <typename T>
+__global__ __launch_bounds__(kConvThreads) void k_convective_batch(const CaseDev* __restrict__ cases,
+                                                                   const int* __restrict__ list, int ordered) {
+    __shared__ double red[kConvThreads];
+    __shared__ double uOutS;
+    const CaseDev& c = cases[list[blockIdx.x]];
+    convective_speed_block(static_cast<const T*>(c.src), c.g, c.fibm, c.u0, c.explicit_state, c.ucol,
+                           const_cast<double*>(c.bc.delU),
+                           ordered, red, uOutS);
+}
+
+template <typename T>
 void launch_convective_speed(const T* src, const GridScalars& g, const double* fibm, const double* u0,
                              bool explicit_state, double* ucol, double* delU, bool ordered, hipStream_t s) {
-    k_convective_speed<T><<<1, 1024, 0, s>>>(src, g, fibm, u0, explicit_state ? 1 : 0, ucol, delU, ordered ? 1 : 0);
+    k_convective_speed<T><<<1, kConvThreads, 0, s>>>(src, g, fibm, u0, explicit_state ? 1 : 0, ucol, delU,
+                                                     ordered ? 1 : 0);
+}
+
+template <typename T>
+void launch_bc_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_nbc, hipStream_t s) {
+    if (n <= 0 || max_nbc <= 0) return;
+    const dim3 grid((unsigned)((max_nbc + 255) / 256), (unsigned)n);
+    if (cm) k_bc_batch<T, true><<<grid, 256, 0, s>>>(cases, list);
+    else k_bc_batch<T, false><<<grid, 256, 0, s>>>(cases, list);
+}
+
+template <typename T>
+void launch_convective_batch(const CaseDev* cases, const int* list, int n, bool ordered, hipStream_t s) {
+    if (n <= 0) return;
+    k_convective_batch<T><<<(unsigned)n, kConvThreads, 0, s>>>(cases, list, ordered ? 1 : 0);
 }
 
 // f = f_eq(rho, u) at every site (Grid.cpp:1048-1061).
@@ -371,7 +421,9 @@ void launch_init_moments(double* rho0, double* u0, const GridScalars& g, int mod
     template void launch_init_equilibrium<T>(T*, const GridScalars&, const double*, const double*, hipStream_t);     \
     template void launch_materialise<T>(const T*, const GridScalars&, const double*, const double*, const double*,   \
                                         bool, double*, double*, double*, double*, hipStream_t);                      \
-    template void launch_upload<T>(T*, const GridScalars&, const double*, hipStream_t);
+    template void launch_upload<T>(T*, const GridScalars&, const double*, hipStream_t);                             \
+    template void launch_bc_batch<T>(bool, const CaseDev*, const int*, int, int, hipStream_t);                       \
+    template void launch_convective_batch<T>(const CaseDev*, const int*, int, bool, hipStream_t);
 LIFE_INST(double)
 LIFE_INST(float)
 

@@ -15,115 +15,13 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "d2q9_f32.h"
 #include "dev_util.h"
 #include "launch.h"
 
 namespace life {
 
-#define LIFE_DF __device__ __forceinline__
-
 using v4f = float __attribute__((ext_vector_type(4)));
-
-LIFE_DF float w32(int v) { return v == 0 ? 4.0f / 9.0f : (v < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
-
-// BGK + Guo on deviations.
-LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rho, float ux, float uy, float Fx,
-                           float Fy, float go[kNV]) {
-    const float usq = 1.5f * (ux * ux + uy * uy);
-    const float cF = 1.0f - 0.5f * omega;
-#pragma unroll
-    for (int v = 0; v < kNV; v++) {
-        const float cx = (float)kCX[v], cy = (float)kCY[v];
-        const float cu = cx * ux + cy * uy;
-        const float eq = w32(v) * (drho + rho * (3.0f * cu + 4.5f * cu * cu - usq));
-        const float fv = 3.0f * w32(v) * ((cx - ux) * Fx + (cy - uy) * Fy + 3.0f * cu * (cx * Fx + cy * Fy));
-        go[v] = g[v] + omega * (eq - g[v]) + cF * fv;
-    }
-}
-
-// Central moments on deviations (De Rosis 2019 basis, Grid.cpp:106-223).
-LIFE_DF void collide_cm32(float omega, float cs2, float cs4, const float g[kNV], float drho, float ux, float uy,
-                          float Fx, float Fy, float go[kNV]) {
-    float k4p = ux * ux - uy * uy, k5p = ux * uy;  // sum_v w_v (...) in closed form
-#pragma unroll
-    for (int v = 0; v < kNV; v++) {
-        const float dx = (float)kCX[v] - ux, dy = (float)kCY[v] - uy;
-        k4p += g[v] * (dx * dx - dy * dy);
-        k5p += g[v] * dx * dy;
-    }
-    const float k1 = 0.5f * Fx, k2 = 0.5f * Fy;
-    const float k4 = (1.0f - omega) * k4p, k5 = (1.0f - omega) * k5p;
-    const float k6 = 0.5f * Fy * cs2, k7 = 0.5f * Fx * cs2;
-    const float X = ux, Y = uy, XX = ux * ux, YY = uy * uy, XY = ux * uy;
-    const float XYY = X * YY, XXY = XX * Y;
-    // M^-1 columns for k1, k2, k4, k5, k6, k7
-    const float m1[kNV] = {2.0f * XYY - 2.0f * X,
-                           X - XYY - 0.5f * YY + 0.5f,
-                           X - XYY + 0.5f * YY - 0.5f,
-                           -XYY - XY,
-                           -XYY + XY,
-                           0.25f * Y + 0.5f * XY + 0.5f * XYY + 0.25f * YY,
-                           0.25f * Y - 0.5f * XY + 0.5f * XYY - 0.25f * YY,
-                           0.5f * XYY - 0.5f * XY - 0.25f * Y + 0.25f * YY,
-                           0.5f * XY - 0.25f * Y + 0.5f * XYY - 0.25f * YY};
-    const float m2[kNV] = {2.0f * XXY - 2.0f * Y,
-                           -XXY - XY,
-                           -XXY + XY,
-                           Y - XXY - 0.5f * XX + 0.5f,
-                           Y - XXY + 0.5f * XX - 0.5f,
-                           0.25f * X + 0.5f * XY + 0.5f * XXY + 0.25f * XX,
-                           0.25f * X - 0.5f * XY + 0.5f * XXY - 0.25f * XX,
-                           0.5f * XY - 0.25f * X + 0.5f * XXY - 0.25f * XX,
-                           0.5f * XXY - 0.5f * XY - 0.25f * X + 0.25f * XX};
-    const float m4[kNV] = {0.5f * YY - 0.5f * XX,
-                           0.25f * XX + 0.25f * X - 0.25f * YY + 0.25f,
-                           0.25f * XX - 0.25f * X - 0.25f * YY + 0.25f,
-                           0.25f * XX - 0.25f * YY - 0.25f * Y - 0.25f,
-                           0.25f * XX - 0.25f * YY + 0.25f * Y - 0.25f,
-                           -0.125f * XX - 0.125f * X + 0.125f * YY + 0.125f * Y,
-                           -0.125f * XX + 0.125f * X + 0.125f * YY - 0.125f * Y,
-                           -0.125f * XX - 0.125f * X + 0.125f * YY - 0.125f * Y,
-                           -0.125f * XX + 0.125f * X + 0.125f * YY + 0.125f * Y};
-    const float m5[kNV] = {4.0f * XY,
-                           -Y - 2.0f * XY,
-                           Y - 2.0f * XY,
-                           -X - 2.0f * XY,
-                           X - 2.0f * XY,
-                           0.5f * X + 0.5f * Y + XY + 0.25f,
-                           XY - 0.5f * Y - 0.5f * X + 0.25f,
-                           0.5f * Y - 0.5f * X + XY - 0.25f,
-                           0.5f * X - 0.5f * Y + XY - 0.25f};
-    const float m6[kNV] = {2.0f * Y, -Y, -Y, -Y - 0.5f, 0.5f - Y,
-                           0.5f * Y + 0.25f, 0.5f * Y - 0.25f, 0.5f * Y - 0.25f, 0.5f * Y + 0.25f};
-    const float m7[kNV] = {2.0f * X, -X - 0.5f, 0.5f - X, -X, -X,
-                           0.5f * X + 0.25f, 0.5f * X - 0.25f, 0.5f * X + 0.25f, 0.5f * X - 0.25f};
-#pragma unroll
-    for (int v = 0; v < kNV; v++) {
-        const float cx = (float)kCX[v], cy = (float)kCY[v];
-        const float bx = 9.0f * (cx * cx) * XX + 6.0f * cx * X - 3.0f * XX;
-        const float by = 9.0f * (cy * cy) * YY + 6.0f * cy * Y - 3.0f * YY;
-        const float dev = w32(v) * (0.5f * (bx + by) + 0.25f * bx * by);  // feqCM(1,u) - w
-        const float feq1 = w32(v) + dev;
-        go[v] = dev + drho * feq1 + m1[v] * k1 + m2[v] * k2 + m4[v] * k4 + m5[v] * k5 + m6[v] * k6 + m7[v] * k7;
-    }
-}
-
-template <bool CM>
-LIFE_DF void collide32(const GridScalars& g, const float gs[kNV], float drho, float ux, float uy, float Fx, float Fy,
-                       float go[kNV]) {
-    if (CM) collide_cm32((float)g.omega, (float)g.cs2, (float)g.cs4, gs, drho, ux, uy, Fx, Fy, go);
-    else collide_bgk32((float)g.omega, gs, drho, 1.0f + drho, ux, uy, Fx, Fy, go);
-}
-
-LIFE_DF void moments32(const float gs[kNV], float Fx, float Fy, float& drho, float& ux, float& uy) {
-    const float dr = ((gs[0] + gs[1]) + (gs[2] + gs[3])) + ((gs[4] + gs[5]) + (gs[6] + gs[7])) + gs[8];
-    const float jx = (gs[1] - gs[2]) + (gs[5] - gs[6]) + (gs[7] - gs[8]);
-    const float jy = (gs[3] - gs[4]) + (gs[5] - gs[6]) + (gs[8] - gs[7]);
-    const float inv = 1.0f / (1.0f + dr);
-    drho = dr;
-    ux = (jx + 0.5f * Fx) * inv;
-    uy = (jy + 0.5f * Fy) * inv;
-}
 
 // Store with or without the non-temporal (streaming) hint.
 template <bool NT>
@@ -175,13 +73,11 @@ __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, f
     st32<B>(&dst[8 * S + rm + jp], go[8]);
 }
 
-// Bulk stream-collide with LDS-realigned stores (see push_store_realigned).
-template <bool CM, bool XCD = true>
-__global__ __launch_bounds__(kPushBlock) void k_push32_lds(const float* __restrict__ src, float* __restrict__ dst,
-                                                           GridScalars g, unsigned nblocks) {
-    __shared__ PushStage<float> st;
-    const long long base = (long long)(XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x) * kPushBlock;
-    const int t = threadIdx.x;
+// Bulk stream-collide body of one block of kPushBlock sites from `base`.
+// SP/DP: float pointers, generic or in the global address space.
+template <bool CM, typename SP, typename DP>
+__device__ __forceinline__ void push32_lds_block(SP __restrict__ src, DP __restrict__ dst, const GridScalars& g,
+                                                 long long base, int t, PushStage<float>& st) {
     const long long id = base + t;
     const bool valid = id < g.N;
     const long long idc = valid ? id : 0;
@@ -196,6 +92,46 @@ __global__ __launch_bounds__(kPushBlock) void k_push32_lds(const float* __restri
     moments32(gs, Fx, Fy, drho, ux, uy);
     collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
     push_store_realigned<float, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, go, valid, st);
+}
+
+// Bulk stream-collide with LDS-realigned stores (see push_store_realigned).
+template <bool CM, bool XCD = true>
+__global__ __launch_bounds__(kPushBlock) void k_push32_lds(const float* __restrict__ src, float* __restrict__ dst,
+                                                           GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<float> st;
+    const long long base = (long long)(XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x) * kPushBlock;
+    push32_lds_block<CM>(src, dst, g, base, threadIdx.x, st);
+}
+
+// Serving: one launch over the blocks of every admitted case of one
+// collision operator.  A case's first step after admission takes its
+// rho/u/force_ibm as stored (the explicit path of k_push32).
+template <bool CM>
+__global__ __launch_bounds__(kPushBlock) void k_push32_batch(const CaseDev* __restrict__ cases,
+                                                             const CaseBlock* __restrict__ blocks) {
+    __shared__ PushStage<float> st;
+    const CaseBlock b = blocks[blockIdx.x];
+    const CaseDev& c = cases[b.c];
+    const long long base = (long long)b.blk * kPushBlock;
+    if (c.explicit_state) {
+        const long long id = base + threadIdx.x;
+        if (id < c.g.N)
+            push32_explicit_site<CM>(static_cast<const float*>(c.src), static_cast<float*>(c.dst), c.g, c.rho0, c.u0,
+                                     c.fibm, id);
+    } else {
+        // table pointers are global memory: typed so, the bulk loads and
+        // stores are global (not flat) instructions
+        push32_lds_block<CM>((gptr<const float>)c.src, (gptr<float>)c.dst, c.g, base, threadIdx.x, st);
+    }
+}
+
+template <bool CM>
+__global__ __launch_bounds__(256) void k_support_push32_batch(const CaseDev* __restrict__ cases,
+                                                              const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c.explicit_state || c.m.n <= 0 || k >= *c.sc.uniq_n) return;
+    support_push32_site<CM>(static_cast<const float*>(c.src), static_cast<float*>(c.dst), c.g, c.fibm, c.sc.uniq[k]);
 }
 
 // Vector form of k_push32 for Ny % 4 == 0: each lane owns four consecutive
@@ -292,23 +228,7 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
                                                         const int* __restrict__ count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *count) return;
-    const long long S = g.S, id = sites[k];
-    const int Ny = g.Ny, Nx = g.Nx;
-    const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
-    float gs[kNV], go[kNV];
-#pragma unroll
-    for (int v = 0; v < kNV; v++) gs[v] = src[v * S + id];
-    const float Fx = (float)(g.Fx + fibm[id]), Fy = (float)(g.Fy + fibm[S + id]);
-    float drho, ux, uy;
-    moments32(gs, Fx, Fy, drho, ux, uy);
-    collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
-#pragma unroll
-    for (int v = 0; v < kNV; v++) {
-        int ii = i + kCX[v], jj = j + kCY[v];
-        ii = ii < 0 ? Nx - 1 : (ii >= Nx ? 0 : ii);
-        jj = jj < 0 ? Ny - 1 : (jj >= Ny ? 0 : jj);
-        dst[v * S + (long long)ii * Ny + jj] = go[v];
-    }
+    support_push32_site<CM>(src, dst, g, fibm, sites[k]);
 }
 
 // Launch configuration of the bulk stream-collide kernel, read once from
@@ -390,6 +310,19 @@ void launch_support_push32(const float* src, float* dst, const GridScalars& g, c
     const unsigned nb = (unsigned)((max_sites + 255) / 256);
     if (g.cm) k_support_push32<true><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
     else k_support_push32<false><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+}
+
+void launch_push32_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    if (cm) k_push32_batch<true><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+    else k_push32_batch<false><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+}
+
+void launch_support_push32_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s) {
+    if (n <= 0 || max_sites <= 0) return;
+    const dim3 grid((unsigned)((max_sites + 255) / 256), (unsigned)n);
+    if (cm) k_support_push32_batch<true><<<grid, 256, 0, s>>>(cases, list);
+    else k_support_push32_batch<false><<<grid, 256, 0, s>>>(cases, list);
 }
 
 }  // namespace life

@@ -61,12 +61,10 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
 
 // Bulk step with LDS-realigned stores (push_store_realigned): the same
 // values reach the same addresses as k_push64, with aligned wavefront stores.
-template <bool CM>
-__global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
-                                                           GridScalars g, unsigned nblocks) {
-    __shared__ PushStage<double> st;
-    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * kPushBlock;
-    const int t = threadIdx.x;
+// SP/DP: double pointers, generic or in the global address space.
+template <bool CM, typename SP, typename DP>
+__device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restrict__ dst, const GridScalars& g,
+                                                 long long base, int t, PushStage<double>& st) {
     const long long id = base + t;
     const bool valid = id < g.N;
     const long long idc = valid ? id : 0;
@@ -82,26 +80,20 @@ __global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restr
     push_store_realigned<double, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
 }
 
-// Redo the push of the support sites of the last spread, where the stored
-// u includes force_ibm and the Guo/central-moment forcing uses
-// force_xy + force_ibm.  sites[0..*count) are lattice ids.
 template <bool CM>
-__global__ __launch_bounds__(256) void k_support_push64(const double* __restrict__ src, double* __restrict__ dst,
-                                                        GridScalars g, const double* __restrict__ fibm,
-                                                        const int* __restrict__ sites,
-                                                        const int* __restrict__ count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *count) return;
+__global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
+                                                           GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<double> st;
+    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * kPushBlock;
+    push64_lds_block<CM>(src, dst, g, base, threadIdx.x, st);
+}
+
+// Push of one site with periodic wrap (Grid.cpp:226-244).
+__device__ __forceinline__ void push_site64(double* __restrict__ dst, const GridScalars& g, long long id,
+                                            const double fo[kNV]) {
     const long long S = g.S;
-    const long long id = sites[k];
     const int Ny = g.Ny, Nx = g.Nx;
     const int i = (int)(id / Ny), j = (int)(id - (long long)i * Ny);
-    double fs[kNV], fo[kNV];
-    load_site(src, S, id, fs);
-    const double Fx = g.Fx + fibm[id], Fy = g.Fy + fibm[S + id];
-    double rho, ux, uy;
-    moments64(fs, Fx, Fy, rho, ux, uy);
-    collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
 #pragma unroll
     for (int v = 0; v < kNV; v++) {
         int ii = i + kCX[v], jj = j + kCY[v];
@@ -109,6 +101,66 @@ __global__ __launch_bounds__(256) void k_support_push64(const double* __restrict
         jj = jj < 0 ? Ny - 1 : (jj >= Ny ? 0 : jj);
         dst[v * S + (long long)ii * Ny + jj] = fo[v];
     }
+}
+
+// Serving: one launch over the blocks of every admitted case of one
+// collision operator; a case's first step takes rho/u/force_ibm as stored.
+template <bool CM>
+__global__ __launch_bounds__(kPushBlock) void k_push64_batch(const CaseDev* __restrict__ cases,
+                                                             const CaseBlock* __restrict__ blocks) {
+    __shared__ PushStage<double> st;
+    const CaseBlock b = blocks[blockIdx.x];
+    const CaseDev& c = cases[b.c];
+    const long long base = (long long)b.blk * kPushBlock;
+    const double* src = static_cast<const double*>(c.src);
+    double* dst = static_cast<double*>(c.dst);
+    if (c.explicit_state) {
+        const long long id = base + threadIdx.x;
+        if (id >= c.g.N) return;
+        double fs[kNV], fo[kNV];
+        load_site(src, c.g.S, id, fs);
+        const double Fx = c.g.Fx + c.fibm[id], Fy = c.g.Fy + c.fibm[c.g.S + id];
+        collide64<CM>(c.g, fs, c.rho0[id], c.u0[2 * id], c.u0[2 * id + 1], Fx, Fy, fo);
+        push_site64(dst, c.g, id, fo);
+    } else {
+        push64_lds_block<CM>((gptr<const double>)c.src, (gptr<double>)c.dst, c.g, base, threadIdx.x, st);
+    }
+}
+
+// Redo the push of the support sites of the last spread, where the stored
+// u includes force_ibm and the Guo/central-moment forcing uses
+// force_xy + force_ibm.  sites[0..*count) are lattice ids.
+template <bool CM>
+__device__ __forceinline__ void support_push64_site(const double* __restrict__ src, double* __restrict__ dst,
+                                                    const GridScalars& g, const double* __restrict__ fibm,
+                                                    long long id) {
+    const long long S = g.S;
+    double fs[kNV], fo[kNV];
+    load_site(src, S, id, fs);
+    const double Fx = g.Fx + fibm[id], Fy = g.Fy + fibm[S + id];
+    double rho, ux, uy;
+    moments64(fs, Fx, Fy, rho, ux, uy);
+    collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
+    push_site64(dst, g, id, fo);
+}
+
+template <bool CM>
+__global__ __launch_bounds__(256) void k_support_push64(const double* __restrict__ src, double* __restrict__ dst,
+                                                        GridScalars g, const double* __restrict__ fibm,
+                                                        const int* __restrict__ sites,
+                                                        const int* __restrict__ count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < *count) support_push64_site<CM>(src, dst, g, fibm, sites[k]);
+}
+
+template <bool CM>
+__global__ __launch_bounds__(256) void k_support_push64_batch(const CaseDev* __restrict__ cases,
+                                                              const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c.explicit_state || c.m.n <= 0 || k >= *c.sc.uniq_n) return;
+    support_push64_site<CM>(static_cast<const double*>(c.src), static_cast<double*>(c.dst), c.g, c.fibm,
+                            c.sc.uniq[k]);
 }
 
 void launch_push64(const double* src, double* dst, const GridScalars& g, const double* rho0, const double* u0,
@@ -139,6 +191,19 @@ void launch_support_push64(const double* src, double* dst, const GridScalars& g,
     const unsigned nb = (unsigned)((max_sites + 255) / 256);
     if (g.cm) k_support_push64<true><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
     else k_support_push64<false><<<nb, 256, 0, s>>>(src, dst, g, fibm, sites, count);
+}
+
+void launch_push64_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    if (cm) k_push64_batch<true><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+    else k_push64_batch<false><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+}
+
+void launch_support_push64_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s) {
+    if (n <= 0 || max_sites <= 0) return;
+    const dim3 grid((unsigned)((max_sites + 255) / 256), (unsigned)n);
+    if (cm) k_support_push64_batch<true><<<grid, 256, 0, s>>>(cases, list);
+    else k_support_push64_batch<false><<<grid, 256, 0, s>>>(cases, list);
 }
 
 }  // namespace life

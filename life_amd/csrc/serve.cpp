@@ -1,0 +1,712 @@
+// Serving engine (life_server_*): continuous batching of independent LIFE
+// cases on one device.
+//
+// Requests (params.h values, geometry.config text, a step count) queue in
+// arrival order and are admitted while a slot is free and their device
+// footprint fits the HBM budget.  Every engine step advances each admitted
+// case by one timestep:
+//  - one table of per-case descriptors (CaseDev) goes to the device;
+//  - the lattice phase of all cases (outlet speed, stream-collide over a
+//    packed (case, block) index, support-site re-push, boundaries) is one
+//    hipGraph launch, captured again only when the set of cases changes;
+//  - the coupling loop (ObjectsClass::objectKernel, Objects.cpp:26-60) runs
+//    in lockstep: the device work of every case still subiterating is one
+//    batched launch per kernel, marker data crosses the bus in one packed
+//    copy each way, and the FP64 FEM runs on host threads, one case each;
+//  - one batched spread.
+// A finished case frees its slot and memory at the end of the step.  Each
+// case goes through the same kernels' arithmetic as life_case_step, so its
+// results do not depend on the cases beside it.
+//
+// Compiled with -ffp-contract=off.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "case_impl.h"
+
+using namespace life;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return 2;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+struct Request {
+    int id = 0;
+    life_params p{};
+    std::string geometry;
+    int nsteps = 0;
+    bool keep_fields = false;
+    life_request_info info{};
+    std::string error;
+    std::vector<double> rho, u, mpos;
+};
+
+struct Slot {
+    Request* req = nullptr;
+    life_case* c = nullptr;
+    int moff = 0;  // first marker in the packed staging arrays
+};
+
+// Device footprint of a case: two population buffers, force_ibm, the
+// initial rho/u, the spread's per-site counters, plus a margin for the
+// boundary and marker arrays.
+double case_bytes(const life_params& p, int prec) {
+    const double N = (double)p.Nx * p.Ny;
+    const double S = std::ceil(N / 256.0) * 256.0;
+    const double T = prec == LIFE_FP64 ? 8.0 : 4.0;
+    return S * (18.0 * T + 16.0 + 24.0) + N * 8.0 + (1 << 20);
+}
+
+template <typename T>
+T* host_pinned(size_t n) {
+    void* p = nullptr;
+    LIFE_HIP(hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault));
+    return static_cast<T*>(p);
+}
+
+// A device array with a pinned host mirror, grown on demand.
+template <typename T>
+struct Mirror {
+    T* d = nullptr;
+    T* h = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        release();
+        cap = std::max<size_t>(n, 2 * cap);
+        d = dalloc<T>(cap);
+        h = host_pinned<T>(cap);
+    }
+    void release() {
+        dfree(d);
+        if (h) (void)hipHostFree(h);
+        d = nullptr;
+        h = nullptr;
+        cap = 0;
+    }
+    void upload(size_t n, hipStream_t s) {
+        if (n) LIFE_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(size_t n, hipStream_t s) {
+        if (n) LIFE_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// Case lists of the lattice phase, fixed while the set of cases is.
+struct LatticePlan {
+    std::vector<CaseBlock> blocks[2];  // BGK, central moments
+    std::vector<int> conv, ibm[2], bc[2];
+    int max_sites[2] = {0, 0}, max_nbc[2] = {0, 0};
+};
+
+}  // namespace
+
+struct life_server {
+    life_server_config cfg{};
+    hipStream_t stream = nullptr;
+    Clock::time_point t0;
+    int next_id = 0;
+    std::map<int, std::unique_ptr<Request>> requests;
+    std::deque<Request*> queue;
+    std::vector<Slot> slots;
+    double used_bytes = 0.0;
+    bool dirty = true;
+    // device tables
+    Mirror<CaseDev> cases;
+    Mirror<CaseBlock> blocks[2];
+    Mirror<int> lists;  // static lists of the lattice phase, back to back
+    Mirror<int> dyn;    // per-subiteration list
+    Mirror<CaseSys> sys;
+    Mirror<int> outside;
+    Mirror<double> stage_in, stage_out;
+    LatticePlan plan;
+    int off_conv = 0, off_ibm[2] = {0, 0}, off_bc[2] = {0, 0}, off_ibm_all = 0, n_ibm_all = 0;
+    int total_markers = 0, max_markers = 0, max_bodies = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // statistics
+    double site_updates = 0.0, steps = 0.0, busy = 0.0, captures = 0.0, peak_running = 0.0, peak_bytes = 0.0;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+void drop_graph(life_server* s) {
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+}
+
+void fail(life_server* s, Slot& sl, const std::string& msg) {
+    sl.req->info.status = LIFE_REQ_FAILED;
+    sl.req->error = msg;
+    (void)s;
+}
+
+// Admit queued requests in arrival order while a slot and budget remain.
+void admit(life_server* s) {
+    while (!s->queue.empty() && (int)s->slots.size() < s->cfg.max_cases) {
+        Request* r = s->queue.front();
+        const double bytes = case_bytes(r->p, s->cfg.precision);
+        if (!s->slots.empty() && s->used_bytes + bytes > s->cfg.hbm_budget) break;
+        s->queue.pop_front();
+        r->info.t_admit = s->now();
+        r->info.bytes = bytes;
+        life_case* c = nullptr;
+        if (life_case_create(&r->p, r->geometry.c_str(), s->cfg.precision, s->cfg.device, &c)) {
+            r->info.status = LIFE_REQ_FAILED;
+            r->error = life_last_error();
+            r->info.t_finish = s->now();
+            continue;
+        }
+        // force_ibm is zero from creation and no spread has run, so the
+        // first spread needs no full clear
+        c->g->dense_clear_next = false;
+        r->info.status = LIFE_REQ_RUNNING;
+        r->info.markers = c->nm;
+        s->slots.push_back(Slot{r, c, 0});
+        s->used_bytes += bytes;
+        s->dirty = true;
+    }
+    s->peak_running = std::max(s->peak_running, (double)s->slots.size());
+    s->peak_bytes = std::max(s->peak_bytes, s->used_bytes);
+}
+
+// Case lists and packed indices for the current set of cases.
+void rebuild(life_server* s) {
+    LatticePlan& P = s->plan;
+    P = LatticePlan{};
+    s->total_markers = s->max_markers = s->max_bodies = 0;
+    std::vector<int> ibm_all;
+    for (int q = 0; q < (int)s->slots.size(); q++) {
+        Slot& sl = s->slots[q];
+        life_grid* g = sl.c->g;
+        const int cm = g->g.cm ? 1 : 0;
+        const long long nb = (g->g.N + 255) / 256;
+        for (long long b = 0; b < nb; b++) P.blocks[cm].push_back(CaseBlock{q, (int)b});
+        if (g->convective) P.conv.push_back(q);
+        if (g->nbc > 0) {
+            P.bc[cm].push_back(q);
+            P.max_nbc[cm] = std::max(P.max_nbc[cm], g->nbc);
+        }
+        if (g->md.n > 0) {
+            P.ibm[cm].push_back(q);
+            P.max_sites[cm] = std::max(P.max_sites[cm], g->sc.cap);
+        }
+        if (sl.c->hasIBM) ibm_all.push_back(q);
+        sl.moff = s->total_markers;
+        s->total_markers += sl.c->nm;
+        s->max_markers = std::max(s->max_markers, sl.c->nm);
+        s->max_bodies = std::max(s->max_bodies, g->nbodies);
+    }
+    for (int k = 0; k < 2; k++) {
+        s->blocks[k].reserve(P.blocks[k].size());
+        std::copy(P.blocks[k].begin(), P.blocks[k].end(), s->blocks[k].h);
+        s->blocks[k].upload(P.blocks[k].size(), s->stream);
+    }
+    std::vector<int> all;
+    auto put = [&](const std::vector<int>& v) {
+        const int off = (int)all.size();
+        all.insert(all.end(), v.begin(), v.end());
+        return off;
+    };
+    s->off_conv = put(P.conv);
+    for (int k = 0; k < 2; k++) {
+        s->off_ibm[k] = put(P.ibm[k]);
+        s->off_bc[k] = put(P.bc[k]);
+    }
+    s->off_ibm_all = put(ibm_all);
+    s->n_ibm_all = (int)ibm_all.size();
+    s->lists.reserve(all.size());
+    std::copy(all.begin(), all.end(), s->lists.h);
+    s->lists.upload(all.size(), s->stream);
+    s->cases.reserve(s->slots.size());
+    s->stage_in.reserve(4 * (size_t)s->total_markers);
+    s->stage_out.reserve(3 * (size_t)s->total_markers);
+    s->dyn.reserve(s->slots.size());
+    LIFE_HIP(hipStreamSynchronize(s->stream));  // host list buffers are reused next time
+    drop_graph(s);
+    s->dirty = false;
+}
+
+// Descriptor of every case for this step (ramp of step t+1, current buffers).
+void fill_table(life_server* s) {
+    for (int q = 0; q < (int)s->slots.size(); q++) {
+        const Slot& sl = s->slots[q];
+        life_grid* g = sl.c->g;
+        CaseDev& d = s->cases.h[q];
+        d.g = g->g;
+        d.src = g->f[g->cur];
+        d.dst = g->f[1 - g->cur];
+        d.rho0 = g->rho0;
+        d.u0 = g->u0;
+        d.fibm = g->fibm;
+        d.explicit_state = g->explicit_state ? 1 : 0;
+        d.convective = g->convective ? 1 : 0;
+        d.bc = BCArgs{g->bc, g->nbc, g->u_in, g->rho_in, g->delU, ramp_coefficient(g->p, g->cs.Dt, sl.c->t + 1)};
+        d.ucol = g->ucol;
+        d.m = g->md;
+        d.sc = g->sc;
+        d.nbodies = g->nbodies;
+        d.moff = sl.moff;
+        d.eps = EpsDev{g->eps_nsys, g->eps_first, g->eps_n, g->eps_off, g->eps_outside, g->eps_band, g->eps_y};
+    }
+    s->cases.upload(s->slots.size(), s->stream);
+}
+
+// GridClass::lbmKernel of every case (Grid.cpp:36-100), same stage order
+// as lbm_step: outlet speed, stream-collide, support sites, boundaries.
+void launch_lattice(life_server* s) {
+    const LatticePlan& P = s->plan;
+    const bool f64 = s->cfg.precision == LIFE_FP64;
+    hipStream_t st = s->stream;
+    const int* L = s->lists.d;
+    if (!P.conv.empty()) {
+        if (f64) launch_convective_batch<double>(s->cases.d, L + s->off_conv, (int)P.conv.size(), true, st);
+        else launch_convective_batch<float>(s->cases.d, L + s->off_conv, (int)P.conv.size(), false, st);
+    }
+    for (int k = 0; k < 2; k++) {
+        const int nb = (int)P.blocks[k].size();
+        if (f64) launch_push64_batch(k == 1, s->cases.d, s->blocks[k].d, nb, st);
+        else launch_push32_batch(k == 1, s->cases.d, s->blocks[k].d, nb, st);
+    }
+    for (int k = 0; k < 2; k++) {
+        const int n = (int)P.ibm[k].size();
+        if (f64) launch_support_push64_batch(k == 1, s->cases.d, L + s->off_ibm[k], n, P.max_sites[k], st);
+        else launch_support_push32_batch(k == 1, s->cases.d, L + s->off_ibm[k], n, P.max_sites[k], st);
+    }
+    for (int k = 0; k < 2; k++) {
+        const int n = (int)P.bc[k].size();
+        if (f64) launch_bc_batch<double>(k == 1, s->cases.d, L + s->off_bc[k], n, P.max_nbc[k], st);
+        else launch_bc_batch<float>(k == 1, s->cases.d, L + s->off_bc[k], n, P.max_nbc[k], st);
+    }
+    LIFE_HIP(hipGetLastError());
+}
+
+void lattice_phase(life_server* s) {
+    if (!s->cfg.use_graph) {
+        launch_lattice(s);
+        return;
+    }
+    if (!s->exec) {
+        LIFE_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_lattice(s);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        LIFE_HIP(hipStreamEndCapture(s->stream, &s->graph));
+        LIFE_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+        s->captures += 1.0;
+    }
+    LIFE_HIP(hipGraphLaunch(s->exec, s->stream));
+}
+
+// Upload the host marker positions/velocities of the listed cases.
+void markers_in(life_server* s, const std::vector<int>& list) {
+    if (list.empty()) return;
+    for (int q : list) {
+        const Slot& sl = s->slots[q];
+        const life_case* c = sl.c;
+        double* o = s->stage_in.h + 4 * (size_t)sl.moff;
+        for (int k = 0; k < c->nm; k++) {
+            o[4 * k] = c->pos[2 * k];
+            o[4 * k + 1] = c->pos[2 * k + 1];
+            o[4 * k + 2] = c->vel[2 * k];
+            o[4 * k + 3] = c->vel[2 * k + 1];
+        }
+    }
+    s->stage_in.upload(4 * (size_t)s->total_markers, s->stream);
+    std::copy(list.begin(), list.end(), s->dyn.h);
+    s->dyn.upload(list.size(), s->stream);
+    launch_markers_in_batch(s->cases.d, s->dyn.d, (int)list.size(), s->max_markers, s->stage_in.d, s->stream);
+    LIFE_HIP(hipGetLastError());
+}
+
+void launch_interpolate_batch_any(bool f64, life_server* s, const int* list, int n) {
+    if (f64) launch_interpolate_batch<double>(s->cases.d, list, n, s->max_markers, s->stream);
+    else launch_interpolate_batch<float>(s->cases.d, list, n, s->max_markers, s->stream);
+    LIFE_HIP(hipGetLastError());
+}
+
+// fn(slot) for the listed cases on host threads.  Cases are independent
+// (disjoint host state); a case whose host work throws is marked failed
+// and leaves the lockstep loop.
+template <typename Fn>
+void for_cases(life_server* s, std::vector<int>& list, Fn&& fn) {
+    const int n = (int)list.size();
+    std::vector<std::string> err(n);
+#pragma omp parallel for schedule(dynamic, 1) if (n > 1)
+    for (int k = 0; k < n; k++) {
+        try {
+            fn(s->slots[list[k]]);
+        } catch (const std::exception& e) {
+            err[k] = e.what();
+            if (err[k].empty()) err[k] = "host error";
+        }
+    }
+    std::vector<int> keep;
+    for (int k = 0; k < n; k++) {
+        if (err[k].empty()) keep.push_back(list[k]);
+        else fail(s, s->slots[list[k]], err[k]);
+    }
+    list.swap(keep);
+}
+
+// ObjectsClass::objectKernel (Objects.cpp:26-60) of every case with bodies.
+void coupling_phase(life_server* s) {
+    std::vector<int> flex;
+    for (int q = 0; q < (int)s->slots.size(); q++) {
+        life_case* c = s->slots[q].c;
+        if (!c->hasIBM) continue;
+        c->subIt = 0;
+        c->forces_on_host = false;
+        if (c->hasFlex) flex.push_back(q);
+    }
+    if (s->n_ibm_all == 0) return;
+    const bool f64 = s->cfg.precision == LIFE_FP64;
+    hipStream_t st = s->stream;
+    std::vector<int> active = flex;
+    bool first = true;
+    std::vector<CaseSys> sysl;
+    std::vector<std::pair<int, int>> dense;  // (slot, first marker of a system) needing the dense path
+    std::vector<int> dense_n;
+    while (true) {
+        sysl.clear();
+        dense.clear();
+        dense_n.clear();
+        if (!active.empty()) {
+            // recomputeObjectVals: predictor / relaxation on the host, then
+            // supports, ds and epsilon on the device
+            for_cases(s, active, [&](Slot& sl) { case_recompute_host(sl.c); });
+            markers_in(s, active);
+            const int na = (int)active.size();
+            launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
+            launch_compute_ds_batch(s->cases.d, s->dyn.d, na, s->max_bodies, 1, st);
+            int max_n = 0;
+            for (int q : active) {
+                life_case* c = s->slots[q].c;
+                life_grid* g = c->g;
+                if (c->p.uni_epsilon) {
+                    dense.push_back({q, 0});
+                    dense_n.push_back(c->nm);
+                    continue;
+                }
+                for (int k = 0; k < g->eps_nsys; k++) {
+                    sysl.push_back(CaseSys{q, k});
+                    max_n = std::max(max_n, g->eps_n_h[k]);
+                }
+            }
+            if (!sysl.empty()) {
+                s->sys.reserve(sysl.size());
+                s->outside.reserve(sysl.size());
+                std::copy(sysl.begin(), sysl.end(), s->sys.h);
+                s->sys.upload(sysl.size(), st);
+                launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, st);
+                s->outside.download(sysl.size(), st);
+            }
+        }
+        // ibmKernelInterp: every case with bodies on the first pass, then
+        // the cases still subiterating
+        if (first) {
+            launch_interpolate_batch_any(f64, s, s->lists.d + s->off_ibm_all, s->n_ibm_all);
+        } else {
+            launch_interpolate_batch_any(f64, s, s->dyn.d, (int)active.size());
+        }
+        first = false;
+        if (active.empty()) break;
+        launch_markers_out_batch(s->cases.d, s->dyn.d, (int)active.size(), s->max_markers, s->stage_out.d, st);
+        LIFE_HIP(hipGetLastError());
+        s->stage_out.download(3 * (size_t)s->total_markers, st);
+        LIFE_HIP(hipStreamSynchronize(st));
+        for (size_t k = 0; k < sysl.size(); k++)
+            if (s->outside.h[k]) {
+                life_grid* g = s->slots[sysl[k].c].c->g;
+                dense.push_back({sysl[k].c, g->eps_first_h[sysl[k].k]});
+                dense_n.push_back(g->eps_n_h[sysl[k].k]);
+            }
+        for (int q : active) {
+            const Slot& sl = s->slots[q];
+            life_case* c = sl.c;
+            const double* o = s->stage_out.h + 3 * (size_t)sl.moff;
+            for (int k = 0; k < c->nm; k++) {
+                c->force[2 * k] = o[3 * k];
+                c->force[2 * k + 1] = o[3 * k + 1];
+                c->eps[k] = o[3 * k + 2];
+            }
+        }
+        // systems that are not banded: computeEpsilon's dense solve
+        for (size_t k = 0; k < dense.size(); k++) {
+            life_case* c = s->slots[dense[k].first].c;
+            case_solve_eps_dense(c, dense[k].second, dense_n[k]);
+            LIFE_HIP(hipStreamSynchronize(c->g->stream));
+        }
+        // femKernel on host threads, then the convergence test per case
+        for_cases(s, active, [&](Slot& sl) { case_fem(sl.c); });
+        std::vector<int> next;
+        for (int q : active)
+            if (case_subiterate(s->slots[q].c)) next.push_back(q);
+        active.swap(next);
+        if (active.empty()) break;
+    }
+    // ibmKernelSpread of every case with bodies
+    launch_spread_batch(s->cases.d, s->lists.d + s->off_ibm_all, s->n_ibm_all, s->max_markers, st);
+    LIFE_HIP(hipGetLastError());
+    // the last FEM solve moved the markers: the device takes the final
+    // positions (as life_case_step does after its spread)
+    markers_in(s, flex);
+    for (int q : flex) {
+        life_case* c = s->slots[q].c;
+        c->forces_on_host = true;
+        if (c->subIt == kMaxSubIt) fail(s, s->slots[q], "Subiteration scheme hit 20 iterations");
+    }
+}
+
+// Results of a finished case, then its slot and memory go back.
+void retire(life_server* s, Slot& sl) {
+    Request* r = sl.req;
+    life_case* c = sl.c;
+    if (r->info.status != LIFE_REQ_FAILED) {
+        try {
+            if (c->hasIBM) check_flags(c->g);
+            if (life_case_total_force(c, r->info.total_force)) throw std::runtime_error(life_last_error());
+            if (c->nm > 0) {
+                const int k = c->nm - 1;
+                r->info.tip[0] = c->pos[2 * k];
+                r->info.tip[1] = c->pos[2 * k + 1];
+                r->info.tip[2] = c->vel[2 * k];
+                r->info.tip[3] = c->vel[2 * k + 1];
+            }
+            if (r->keep_fields) {
+                const size_t N = (size_t)c->g->g.N;
+                r->rho.resize(N);
+                r->u.resize(2 * N);
+                if (life_grid_download(c->g, nullptr, r->rho.data(), r->u.data(), nullptr))
+                    throw std::runtime_error(life_last_error());
+                r->mpos = c->pos;
+            }
+            r->info.status = LIFE_REQ_DONE;
+        } catch (const HipError& e) {
+            fail(s, sl, std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        } catch (const std::exception& e) {
+            fail(s, sl, e.what());
+        }
+    }
+    r->info.t_finish = s->now();
+    life_case_destroy(c);
+    sl.c = nullptr;
+    s->used_bytes -= r->info.bytes;
+}
+
+void engine_step(life_server* s) {
+    admit(s);
+    if (s->slots.empty()) return;
+    const double t_begin = s->now();
+    if (s->dirty) rebuild(s);
+    fill_table(s);
+    lattice_phase(s);
+    for (Slot& sl : s->slots) {
+        life_case* c = sl.c;
+        c->t++;
+        c->g->cur = 1 - c->g->cur;
+        c->g->explicit_state = false;
+        s->site_updates += (double)c->g->g.N;
+    }
+    coupling_phase(s);
+    LIFE_HIP(hipStreamSynchronize(s->stream));
+    s->steps += 1.0;
+    for (int q = (int)s->slots.size() - 1; q >= 0; q--) {
+        Slot& sl = s->slots[q];
+        sl.req->info.steps_done = sl.c->t;
+        sl.req->info.subits = sl.c->subIt;
+        if (sl.c->t >= sl.req->nsteps || sl.req->info.status == LIFE_REQ_FAILED) {
+            retire(s, sl);
+            s->slots.erase(s->slots.begin() + q);
+            s->dirty = true;
+        }
+    }
+    s->busy += s->now() - t_begin;
+}
+
+int submit(life_server* s, const life_params* p, const char* geometry, int nsteps, int keep_fields,
+           double t_arrival) {
+    if (!p) throw std::runtime_error("null params");
+    if (nsteps < 1) throw std::runtime_error("nsteps must be positive");
+    std::unique_ptr<Request> r(new Request());
+    r->id = s->next_id++;
+    r->p = *p;
+    r->geometry = geometry ? geometry : "";
+    r->nsteps = nsteps;
+    r->keep_fields = keep_fields != 0;
+    r->info.status = LIFE_REQ_QUEUED;
+    r->info.nsteps = nsteps;
+    r->info.t_arrival = t_arrival;
+    r->info.t_admit = r->info.t_finish = -1.0;
+    r->info.sites = (double)p->Nx * p->Ny;
+    s->queue.push_back(r.get());
+    const int id = r->id;
+    s->requests[id] = std::move(r);
+    return id;
+}
+
+const Request* find(const life_server* s, int id) {
+    auto it = s->requests.find(id);
+    if (it == s->requests.end()) throw std::runtime_error("unknown request id");
+    return it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int life_server_create(const life_server_config* cfg, life_server** out) {
+    *out = nullptr;
+    life_server* s = new life_server();
+    const int rc = guarded([&] {
+        if (!cfg) throw std::runtime_error("null config");
+        if (cfg->precision != LIFE_FP64 && cfg->precision != LIFE_FP32_SHIFTED)
+            throw std::runtime_error("unknown precision");
+        if (cfg->max_cases < 1) throw std::runtime_error("max_cases must be positive");
+        if (!(cfg->hbm_budget > 0.0)) throw std::runtime_error("hbm_budget must be positive");
+        s->cfg = *cfg;
+        LIFE_HIP(hipSetDevice(cfg->device));
+        LIFE_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->t0 = Clock::now();
+    });
+    if (rc) {
+        life_server_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int life_server_destroy(life_server* s) {
+    if (!s) return 0;
+    const int rc = guarded([&] {
+        (void)hipSetDevice(s->cfg.device);
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        for (Slot& sl : s->slots) life_case_destroy(sl.c);
+        s->slots.clear();
+        drop_graph(s);
+        s->cases.release();
+        for (auto& b : s->blocks) b.release();
+        s->lists.release();
+        s->dyn.release();
+        s->sys.release();
+        s->outside.release();
+        s->stage_in.release();
+        s->stage_out.release();
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+    });
+    delete s;
+    return rc;
+}
+
+int life_server_submit(life_server* s, const life_params* p, const char* geometry, int nsteps, int keep_fields,
+                       int* id) {
+    return guarded([&] { *id = submit(s, p, geometry, nsteps, keep_fields, s->now()); });
+}
+
+int life_server_step(life_server* s, int* running, int* queued) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(s->cfg.device));
+        engine_step(s);
+        if (running) *running = (int)s->slots.size();
+        if (queued) *queued = (int)s->queue.size();
+    });
+}
+
+int life_server_run(life_server* s, int n, const life_params* p, const char* const* geometry, const int* nsteps,
+                    const double* arrival, int* ids) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(s->cfg.device));
+        for (int k = 1; k < n; k++)
+            if (arrival[k] < arrival[k - 1]) throw std::runtime_error("arrival times must be ascending");
+        const double base = s->now();
+        int next = 0;
+        while (true) {
+            const double t = s->now() - base;
+            while (next < n && arrival[next] <= t) {
+                ids[next] = submit(s, &p[next], geometry[next], nsteps[next], 0, base + arrival[next]);
+                next++;
+            }
+            if (s->slots.empty() && s->queue.empty()) {
+                if (next >= n) break;
+                std::this_thread::sleep_for(std::chrono::duration<double>(arrival[next] - t));
+                continue;
+            }
+            engine_step(s);
+        }
+    });
+}
+
+int life_server_info(const life_server* s, int id, life_request_info* out) {
+    return guarded([&] { *out = find(s, id)->info; });
+}
+
+const char* life_server_request_error(const life_server* s, int id) {
+    auto it = s->requests.find(id);
+    return it == s->requests.end() ? "unknown request id" : it->second->error.c_str();
+}
+
+int life_server_fields(const life_server* s, int id, double* rho, double* u, double* marker_pos) {
+    return guarded([&] {
+        const Request* r = find(s, id);
+        if (r->info.status != LIFE_REQ_DONE) throw std::runtime_error("request has not finished");
+        if (!r->keep_fields) throw std::runtime_error("request was submitted without keep_fields");
+        if (rho) std::copy(r->rho.begin(), r->rho.end(), rho);
+        if (u) std::copy(r->u.begin(), r->u.end(), u);
+        if (marker_pos) std::copy(r->mpos.begin(), r->mpos.end(), marker_pos);
+    });
+}
+
+int life_server_release(life_server* s, int id) {
+    return guarded([&] {
+        auto it = s->requests.find(id);
+        if (it == s->requests.end()) throw std::runtime_error("unknown request id");
+        const int st = it->second->info.status;
+        if (st != LIFE_REQ_DONE && st != LIFE_REQ_FAILED) throw std::runtime_error("request is still queued or running");
+        s->requests.erase(it);
+    });
+}
+
+int life_server_stats(const life_server* s, double out[6]) {
+    out[0] = s->site_updates;
+    out[1] = s->steps;
+    out[2] = s->busy;
+    out[3] = s->captures;
+    out[4] = s->peak_running;
+    out[5] = s->peak_bytes;
+    return 0;
+}
+
+}  // extern "C"

@@ -42,6 +42,23 @@ def make_params(p):
     return s
 
 
+class ServerConfig(C.Structure):
+    """struct life_server_config (include/life_hip.h)."""
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("max_cases", C.c_int),
+                ("hbm_budget", C.c_double), ("use_graph", C.c_int)]
+
+
+LIFE_REQ_QUEUED, LIFE_REQ_RUNNING, LIFE_REQ_DONE, LIFE_REQ_FAILED = 0, 1, 2, 3
+
+
+class RequestInfo(C.Structure):
+    """struct life_request_info (include/life_hip.h)."""
+    _fields_ = [("status", C.c_int), ("steps_done", C.c_int), ("nsteps", C.c_int), ("subits", C.c_int), ("markers", C.c_int),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_finish", C.c_double),
+                ("sites", C.c_double), ("bytes", C.c_double),
+                ("total_force", C.c_double * 4), ("tip", C.c_double * 4)]
+
+
 # name -> (restype, argtypes)
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -78,6 +95,16 @@ SIGNATURES = {
     "life_case_get_fem": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "life_case_coupling_info": (C.c_int, [_vp, _dp]),
     "life_case_total_force": (C.c_int, [_vp, _dp]),
+    "life_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "life_server_destroy": (C.c_int, [_vp]),
+    "life_server_submit": (C.c_int, [_vp, C.POINTER(LifeParams), C.c_char_p, C.c_int, C.c_int, _ip]),
+    "life_server_step": (C.c_int, [_vp, _ip, _ip]),
+    "life_server_run": (C.c_int, [_vp, C.c_int, C.POINTER(LifeParams), C.POINTER(C.c_char_p), _ip, _dp, _ip]),
+    "life_server_info": (C.c_int, [_vp, C.c_int, C.POINTER(RequestInfo)]),
+    "life_server_request_error": (C.c_char_p, [_vp, C.c_int]),
+    "life_server_fields": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "life_server_release": (C.c_int, [_vp, C.c_int]),
+    "life_server_stats": (C.c_int, [_vp, _dp]),
 }
 
 _lib = None
