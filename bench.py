@@ -193,6 +193,68 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+CONFIGS = {
+    # name: (builder, default steps, oracle parity steps, description)
+    "ldc256": ("lid_driven_cavity", 20000, 200, "lid-driven cavity 256x256, BGK, tau 0.6, lid 0.05"),
+    "cylinder1024": ("cylinder_channel", 20000, 100,
+                     "rigid cylinder D=40 in a 1024x256 channel, central moments, Re 100, parabolic inlet"),
+    "fsi2048": ("flexible_fsi", 4000, 30,
+                "cylinder D=80 + flexible plate L=280 h=16 (40 elements) in 2048x410, central moments, Re 100"),
+}
+
+
+def bench_config(args):
+    """One of the reference-style validation configurations on its full
+    lattice: FP64 (plain) and FP32-shifted runs of the same steps, MLUPS of
+    each and the FP32 results against the FP64 ones.  (FP64 parity with the
+    oracle on these full lattices is a GPU test, tests/test_gpu_configs.py.)"""
+    import numpy as np
+    from life_amd import config as cfg
+    from life_amd import lib
+    from life_amd.case import Case
+
+    builder, dsteps, _, desc = CONFIGS[args.workload]
+    steps = args.steps if args.steps_given else dsteps
+    p, g = getattr(cfg, builder)()
+    N = p.Nx * p.Ny
+    runs = {}
+    for name, prec in (("f64", lib.LIFE_FP64), ("f32", lib.LIFE_FP32_SHIFTED)):
+        c = Case(p, g, precision=prec)
+        t0 = time.perf_counter()
+        c.step(steps)
+        wall = time.perf_counter() - t0
+        _, rho, u, _ = c.fields()
+        r = {"mlups": N * steps / wall / 1e6, "wall_s": wall, "rho": rho, "u": u}
+        if c.n_markers:
+            r["force"] = c.total_force()
+            r["tip"] = c.markers()["pos"].reshape(-1, 2)[-1].copy()
+        runs[name] = r
+        c.close()
+        print("bench: %s %s %d steps in %.2f s" % (args.workload, name, steps, wall), file=sys.stderr, flush=True)
+    a, b = runs["f32"], runs["f64"]
+    res = {"u_l2_rel_f32_vs_f64": float(np.linalg.norm(a["u"] - b["u"]) / np.linalg.norm(b["u"])),
+           "rho_max_abs_f32_vs_f64": float(np.abs(a["rho"] - b["rho"]).max())}
+    if args.workload == "ldc256":
+        m = p.Nx // 2
+        res["centreline_u_f64"] = b["u"][m, :, 0][::16].tolist()
+        res["centreline_v_f64"] = b["u"][:, m, 1][::16].tolist()
+        res["centreline_max_abs_f32_vs_f64"] = float(max(np.abs(a["u"][m, :, 0] - b["u"][m, :, 0]).max(),
+                                                       np.abs(a["u"][:, m, 1] - b["u"][:, m, 1]).max()))
+    if "force" in b:
+        res["force_f64"] = b["force"].tolist()
+        res["force_rel_err_f32"] = float(np.linalg.norm(a["force"][:2] - b["force"][:2]) /
+                                         np.linalg.norm(b["force"][:2]))
+        res["tip_f64"] = b["tip"].tolist()
+        res["tip_err_f32_over_L"] = float(np.linalg.norm(a["tip"] - b["tip"]) / p.ref_L)
+    out = {"metric": "D2Q9 MLUPS (plain FP64, full step incl. coupling)", "value": runs["f64"]["mlups"],
+           "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": runs["f64"]["wall_s"] * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "description": desc, "Nx": p.Nx, "Ny": p.Ny},
+           "mlups_f32_shifted": runs["f32"]["mlups"], "results": res}
+    print(json.dumps(out))
+
+
 def main():
     if "--calibrate-copy" in sys.argv:
         calibrate_copy(1 << 30, 5)
@@ -212,6 +274,7 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=200e9)
     ap.add_argument("--baseline-cases", type=int, default=16)
     args = ap.parse_args()
+    args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs each case on a single MI355X "
               "(multi-GPU sharding is a design note, not an implementation).", file=sys.stderr)
@@ -219,6 +282,9 @@ def main():
 
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload in CONFIGS:
+        bench_config(args)
         return
 
     import numpy as np

@@ -255,72 +255,75 @@ __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __
         w[r] = A0(r, L);
         yw[r] = r < n ? 1.0 : 0.0;
     }
+    // Register slot (kk + r) % R holds row k + r at step k = k0 + kk: the k
+    // loop is unrolled R times so that the slots rotate with constant
+    // indices instead of moving.
     int pl = 0;   // lane holding column k
     int off = L;  // this lane holds column k + off
-    for (int k = 0; k < n; k++) {
-        const double incoming = A0(k + R, L == pl ? k + C : k + off);  // row k+R, after the step
-        const int rmax = min(KB, n - 1 - k);
-        double col[R], a[R];
+    for (int k0 = 0; k0 < n; k0 += R) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            col[r] = bcast64(w[r], pl);
-            a[r] = fabs(col[r]);
-        }
-        // maximum magnitude (tree), then the first row that attains it
-        double mx[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) mx[r] = a[r];
-#pragma unroll
-        for (int h = 1; h < R; h *= 2)
-#pragma unroll
-            for (int r = 0; r + h < R; r += 2 * h) mx[r] = mx[r + h] > mx[r] ? mx[r + h] : mx[r];
-        const double best = mx[0];
-        int p = 0;
-#pragma unroll
-        for (int r = R - 1; r > 0; r--)
-            if (a[r] == best) p = r;
-        if (a[0] == best) p = 0;
-        if (best != 0.0) {
-            if (p != 0) {
-                const double tw = w[0], tc = col[0], ty = yw[0];
+        for (int kk = 0; kk < R; kk++) {
+            const int k = k0 + kk;
+            if (k < n) {
+                auto S = [&](int r) -> int { return (kk + r) % R; };
+                // row k+R enters this lane's column after the step (band
+                // offset q = column - row + KB, always inside the stored band)
+                const int cn = L == pl ? C : off;
+                const double incoming = (L < C && k + R < n && k + cn < n) ? B[(k + R) * W + (cn - 1)] : 0.0;
+                const int rmax = min(KB, n - 1 - k);
+                // first row of maximum magnitude, found by the lane that holds
+                // column k in its own registers
+                double best = fabs(w[S(0)]);
+                int pr = 0;
 #pragma unroll
                 for (int r = 1; r < R; r++)
-                    if (r == p) {
-                        w[0] = w[r];
-                        w[r] = tw;
-                        col[0] = col[r];
-                        col[r] = tc;
-                        yw[0] = yw[r];
-                        yw[r] = ty;
+                    if (r <= rmax) {
+                        const double a = fabs(w[S(r)]);
+                        if (a > best) {
+                            best = a;
+                            pr = r;
+                        }
                     }
-            }
-            const double piv = col[0];
-            double num = col[1];
+                const int p = __builtin_amdgcn_readlane(pr, pl);
+                if (__builtin_amdgcn_readlane(best != 0.0 ? 1 : 0, pl)) {
+                    if (p != 0) {
+                        const double tw = w[S(0)], ty = yw[S(0)];
 #pragma unroll
-            for (int r = 2; r < R; r++) num = L == r ? col[r] : num;
-            const double lv = num / piv;  // lane r: l_r
+                        for (int r = 1; r < R; r++)
+                            if (r == p) {
+                                w[S(0)] = w[S(r)];
+                                w[S(r)] = tw;
+                                yw[S(0)] = yw[S(r)];
+                                yw[S(r)] = ty;
+                            }
+                    }
+                    // multipliers l_r = a(k+r, k) / a(k, k): column k goes to
+                    // lanes 1..KB (lane r gets row r), one division serves all
+                    const double piv = uniform64(w[S(0)], pl);
+                    double num = 0.0;
 #pragma unroll
-            for (int r = 1; r < R; r++)
-                if (r <= rmax) {
-                    const double l = uniform64(lv, r);
-                    const double upd = w[r] - l * w[0];
-                    w[r] = L == pl ? 0.0 : upd;
-                    yw[r] = yw[r] - l * yw[0];
+                    for (int r = 1; r < R; r++) num = L == r ? uniform64(w[S(r)], pl) : num;
+                    const double lv = num / piv;
+#pragma unroll
+                    for (int r = 1; r < R; r++)
+                        if (r <= rmax) {
+                            const double l = uniform64(lv, r);
+                            const double upd = w[S(r)] - l * w[S(0)];
+                            w[S(r)] = L == pl ? 0.0 : upd;
+                            yw[S(r)] = yw[S(r)] - l * yw[S(0)];
+                        }
                 }
+                // row k of U (in place of row k of the band) and the final y_k
+                if (L < C && k + off < n) B[k * W + (off + KB)] = w[S(0)];
+                if (L == 0) y[k] = yw[S(0)];
+                // row k leaves, row k+R takes its slot; column k's lane takes
+                // column k+C, whose entries in rows k+1..k+KB are zeros
+                w[S(0)] = incoming;
+                yw[S(0)] = k + R < n ? 1.0 : 0.0;
+                off = L == pl ? C - 1 : off - 1;
+                pl = pl == C - 1 ? 0 : pl + 1;
+            }
         }
-        // row k of U (in place of row k of the band) and the final y_k
-        if (L < C && k + off < n) B[k * W + (off + KB)] = w[0];
-        if (L == 0) y[k] = yw[0];
-        // drop row k, take in row k+R; column k's lane takes column k+C
-#pragma unroll
-        for (int r = 0; r < R - 1; r++) {
-            w[r] = w[r + 1];
-            yw[r] = yw[r + 1];
-        }
-        w[R - 1] = incoming;
-        yw[R - 1] = k + R < n ? 1.0 : 0.0;
-        off = L == pl ? C - 1 : off - 1;
-        pl = pl == C - 1 ? 0 : pl + 1;
     }
     __syncthreads();
     if (L == 0) {

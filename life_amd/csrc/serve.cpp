@@ -115,6 +115,35 @@ struct Mirror {
     }
 };
 
+// Optional phase timing (environment LIFE_TIMING=1): synchronises the
+// stream at each phase boundary, so it is for diagnosis only.
+struct EngineTimes {
+    bool on = std::getenv("LIFE_TIMING") != nullptr;
+    // admit+rebuild, lattice, host recompute, device recompute+interp+copies,
+    // host FEM, spread, retire
+    double t[7] = {0, 0, 0, 0, 0, 0, 0};
+    double steps = 0, subits = 0;
+    Clock::time_point last;
+    void start(hipStream_t st) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(st));
+        last = Clock::now();
+    }
+    void lap(hipStream_t st, int k) {
+        if (!on) return;
+        LIFE_HIP(hipStreamSynchronize(st));
+        const auto now = Clock::now();
+        t[k] += std::chrono::duration<double, std::milli>(now - last).count();
+        last = now;
+    }
+    void report() const {
+        if (!on || steps == 0) return;
+        std::fprintf(stderr, "life server timing over %.0f engine steps (%.0f subiterations), ms/step: admit %.3f "
+                     "lattice %.3f recompute-host %.3f device %.3f fem %.3f spread %.3f retire %.3f\n", steps, subits,
+                     t[0] / steps, t[1] / steps, t[2] / steps, t[3] / steps, t[4] / steps, t[5] / steps, t[6] / steps);
+    }
+};
+
 // Case lists of the lattice phase, fixed while the set of cases is.
 struct LatticePlan {
     std::vector<CaseBlock> blocks[2];  // BGK, central moments
@@ -147,6 +176,7 @@ struct life_server {
     int total_markers = 0, max_markers = 0, max_bodies = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    EngineTimes times;
     // statistics
     double site_updates = 0.0, steps = 0.0, busy = 0.0, captures = 0.0, peak_running = 0.0, peak_bytes = 0.0;
 
@@ -407,6 +437,7 @@ void coupling_phase(life_server* s) {
             // recomputeObjectVals: predictor / relaxation on the host, then
             // supports, ds and epsilon on the device
             for_cases(s, active, [&](Slot& sl) { case_recompute_host(sl.c); });
+            s->times.lap(st, 2);
             markers_in(s, active);
             const int na = (int)active.size();
             launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
@@ -470,7 +501,10 @@ void coupling_phase(life_server* s) {
             LIFE_HIP(hipStreamSynchronize(c->g->stream));
         }
         // femKernel on host threads, then the convergence test per case
+        s->times.lap(st, 3);
+        s->times.subits += 1;
         for_cases(s, active, [&](Slot& sl) { case_fem(sl.c); });
+        s->times.lap(st, 4);
         std::vector<int> next;
         for (int q : active)
             if (case_subiterate(s->slots[q].c)) next.push_back(q);
@@ -527,12 +561,15 @@ void retire(life_server* s, Slot& sl) {
 }
 
 void engine_step(life_server* s) {
+    s->times.start(s->stream);
     admit(s);
     if (s->slots.empty()) return;
     const double t_begin = s->now();
     if (s->dirty) rebuild(s);
     fill_table(s);
+    s->times.lap(s->stream, 0);
     lattice_phase(s);
+    s->times.lap(s->stream, 1);
     for (Slot& sl : s->slots) {
         life_case* c = sl.c;
         c->t++;
@@ -542,6 +579,8 @@ void engine_step(life_server* s) {
     }
     coupling_phase(s);
     LIFE_HIP(hipStreamSynchronize(s->stream));
+    s->times.lap(s->stream, 5);
+    s->times.steps += 1;
     s->steps += 1.0;
     for (int q = (int)s->slots.size() - 1; q >= 0; q--) {
         Slot& sl = s->slots[q];
@@ -553,6 +592,7 @@ void engine_step(life_server* s) {
             s->dirty = true;
         }
     }
+    s->times.lap(s->stream, 6);
     s->busy += s->now() - t_begin;
 }
 
@@ -614,6 +654,7 @@ int life_server_destroy(life_server* s) {
     const int rc = guarded([&] {
         (void)hipSetDevice(s->cfg.device);
         if (s->stream) (void)hipStreamSynchronize(s->stream);
+        s->times.report();
         for (Slot& sl : s->slots) life_case_destroy(sl.c);
         s->slots.clear();
         drop_graph(s);
