@@ -156,10 +156,12 @@ def canopy(Nx=8192, Ny=4096, n_filaments=64, L=200.0, n_el=50, min_spacing=60.0,
 
 
 def inverted_flag(Nx=512, Ny=256, length_frac=0.25, n_el=20, Re=200.0, kb=0.5, u_in=0.04,
-                  density_ratio=10.0, h=1.0, central_moments=0):
+                  density_ratio=10.0, h=1.0, central_moments=0, initial_deflect=0.0):
     """Inverted flag (the reference's InvertedFlag case): one flexible
     plate clamped at its trailing edge (angle 180 degrees), uniform inflow,
-    convective outlet, velocity top/bottom.  Plate length = length_frac * Ny."""
+    convective outlet, velocity top/bottom.  Plate length = length_frac * Ny.
+    initial_deflect: INITIAL_DEFLECT (fraction of L; the reference's
+    InvertedFlag example uses 0.01)."""
     L = float(int(length_frac * Ny))
     nu = u_in * L / Re
     tau = 3.0 * nu + 0.5
@@ -169,7 +171,8 @@ def inverted_flag(Nx=512, Ny=256, length_frac=0.25, n_el=20, Re=200.0, kb=0.5, u
     y0 = 0.5 * (Ny - 1)
     p = lattice_case(Nx, Ny, tau, wall_left=VELOCITY, wall_right=CONVECTIVE, wall_bottom=VELOCITY,
                      wall_top=VELOCITY, profile=UNIFORM, uxInlet_p=u_in, ux0_p=u_in,
-                     central_moments=central_moments, subTol=1e-5, ref_L=L, ref_U=u_in)
+                     central_moments=central_moments, subTol=1e-5, ref_L=L, ref_U=u_in,
+                     initial_deflect=initial_deflect)
     geom = ("FILAMENT 1 0 %.17g %.17g 0.0 0.0 %.17g %.17g 180.0 FLEXIBLE %d CLAMPED %.17g %.17g\n"
             % (x0, y0, L, h, n_el, density_ratio, E))
     return p, geom

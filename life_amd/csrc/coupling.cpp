@@ -310,7 +310,6 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
         c->p = *p;
         c->cs = derive_scalars(*p);
         c->relax = p->relaxMax;
-        if (p->initial_deflect != 0.0) throw std::runtime_error("INITIAL_DEFLECT is not supported");
         parse_geometry(c, geometry);
         finalise_markers(c);
         c->hasIBM = !c->bodies.empty();
@@ -323,6 +322,9 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
                 c->flex_bodies.push_back(&b);
             }
         }
+        // ObjectsClass::initialDeflect, Objects.cpp:943-947, 964-976
+        if (p->initial_deflect != 0.0)
+            for (Body* b : c->flex_bodies) b->fem->set_initial_deflection(p->initial_deflect, c->view(*b));
         if (life_grid_create(p, precision, device, &c->g)) throw std::runtime_error(life_last_error());
         if (c->hasIBM) {
             std::vector<int> flex(c->bodies.size());

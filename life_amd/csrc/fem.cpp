@@ -387,6 +387,74 @@ void FemBody::dynamic(MarkerView mv) {
     subDen_ = den;
 }
 
+// FEMBodyClass::staticFEM + newtonRaphsonStatic, FEMBody.cpp:448-484:
+// Newton on K dU = R - F (stiffness only), then the markers follow.
+void FemBody::static_fem(MarkerView mv) {
+    itNR_ = 0;
+    do {
+        build_global();
+        for (int k = 0; k < ndof_; k++) vec_[k] = R[k] - F[k];
+        lu_solve_band(K_, nbc_, vec_.data(), delU.data(), W_, y_);
+        for (int k = 0; k < ndof_; k++) U[k] = U[k] + delU[k];
+        update_fem();
+        resNR_ = std::sqrt(dot(delU.data(), delU.data(), ndof_)) / (s_.ref_L * std::sqrt((double)ndof_));
+        itNR_++;
+    } while (resNR_ > 1e-10 && itNR_ < 20);
+    update_ibm(mv);
+}
+
+namespace {
+// 2x2 matrix times vector, accumulating from 0.0 (Utils.h:375-389).
+void mat2_vec(const double T[2][2], const double v[2], double out[2]) {
+    for (int i = 0; i < 2; i++) {
+        double r = 0.0;
+        for (int j = 0; j < 2; j++) r += T[i][j] * v[j];
+        out[i] = r;
+    }
+}
+}  // namespace
+
+// FEMBodyClass::setInitialDeflection, FEMBody.cpp:352-445: a tip load in
+// the body frame, corrected by a bounded proportional feedback until the
+// static tip deflection is -fraction * L0.
+void FemBody::set_initial_deflection(double fraction, MarkerView mv) {
+    const int nd = ndof_;
+    const double deflect = -fraction;
+    const double E = el_[0].E, I = el_[0].I;
+    const double linearForce = 3.0 * E * I * deflect / (L0_ * L0_);
+    const int nSteps = 100;
+    const double dFmax = linearForce / (double)nSteps;
+    const double T[2][2] = {{std::cos(angle0_), -std::sin(angle0_)}, {std::sin(angle0_), std::cos(angle0_)}};
+    const double Tt[2][2] = {{T[0][0], T[1][0]}, {T[0][1], T[1][1]}};
+    double local[2] = {0.0, dFmax}, global[2];
+    mat2_vec(T, local, global);
+    R[nd - 3] = global[0];
+    R[nd - 2] = global[1];
+    const double gain = 1e-3, TOL = 1e-10;
+    const int MAXIT = 10000;
+    int it = 0;
+    double error;
+    do {
+        static_fem(mv);
+        const double dv[2] = {U[nd - 3], U[nd - 2]};
+        double dl[2];
+        mat2_vec(Tt, dv, dl);
+        error = deflect - dl[1] / L0_;
+        double fc = error * gain;
+        if (std::fabs(fc) > std::fabs(dFmax)) fc = (double)((0.0 < fc) - (fc < 0.0)) * std::fabs(dFmax);
+        local[1] += fc;
+        mat2_vec(T, local, global);
+        R[nd - 3] = global[0];
+        R[nd - 2] = global[1];
+        it++;
+        if (it >= MAXIT) throw std::runtime_error("max iterations hit! Try changing the gain...exiting");
+    } while (std::fabs(error) > TOL);
+    itNR_ = 0;
+    resNR_ = 0.0;
+    std::fill(R.begin(), R.end(), 0.0);
+    std::fill(delU.begin(), delU.end(), 0.0);
+}
+
 void FemBody::predictor(int t, MarkerView mv) {
     if (t > 2)
         for (int k = 0; k < ndof_; k++) U[k] = 2.5 * Un[k] - 2.0 * Unm1[k] + 0.5 * Unm2[k];

@@ -43,6 +43,8 @@ class FemBody {
     void predictor(int t, MarkerView mv);         // FEMBody.cpp:259-289
     void relax(double omega, MarkerView mv);      // Objects.cpp:198-210
     void dynamic(MarkerView mv);                  // FEMBody.cpp:26-68
+    // INITIAL_DEFLECT: static tip deflection of -fraction * L0 (FEMBody.cpp:352-484)
+    void set_initial_deflection(double fraction, MarkerView mv);
 
     int ndof() const { return ndof_; }
     int bc_dofs() const { return nbc_; }
@@ -76,6 +78,7 @@ class FemBody {
         double zeta;
     };
 
+    void static_fem(MarkerView mv);    // FEMBody.cpp:448-484
     void update_fem();                 // FEMBody.cpp:198-223
     void update_ibm(MarkerView mv);    // FEMBody.cpp:147-195
     void finish_newmark();             // FEMBody.cpp:129-144

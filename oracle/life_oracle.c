@@ -1117,6 +1117,74 @@ static void fem_dynamic(orc_case *c, fem_t *F) {
     F->subDen = den;
 }
 
+/* FEMBodyClass::staticFEM + newtonRaphsonStatic, FEMBody.cpp:448-484:
+ * Newton on K dU = R - F (stiffness only), then the markers follow. */
+static void fem_static(orc_case *c, fem_t *F) {
+    int nd = F->ndof;
+    double *rhs = xcalloc(nd, sizeof(double));
+    F->itNR = 0;
+    do {
+        fem_build(F);
+        for (int k = 0; k < nd; k++) rhs[k] = F->R[k] - F->F[k];
+        orc_lu_solve(nd, F->nbc, F->K, rhs, F->delU);
+        for (int k = 0; k < nd; k++) F->U[k] = F->U[k] + F->delU[k];
+        fem_update_fem(F);
+        F->resNR = sqrt(vdot(F->delU, F->delU, nd)) / (c->p.ref_L * sqrt((double)nd));
+        F->itNR++;
+    } while (F->resNR > 1e-10 && F->itNR < 20);
+    fem_update_ibm(c, F);
+    free(rhs);
+}
+
+/* 2x2 matrix times vector with the reference's accumulation from 0.0
+ * (Utils.h:375-389). */
+static void mat2_vec(const double T[2][2], const double v[2], double out[2]) {
+    for (int i = 0; i < 2; i++) {
+        double r = 0.0;
+        for (int j = 0; j < 2; j++) r += T[i][j] * v[j];
+        out[i] = r;
+    }
+}
+
+/* FEMBodyClass::setInitialDeflection, FEMBody.cpp:352-445: a tip load in
+ * the body frame, corrected by a bounded proportional feedback until the
+ * static tip deflection is -INITIAL_DEFLECT * L0. */
+static int fem_initial_deflection(orc_case *c, fem_t *F, double fraction) {
+    int nd = F->ndof;
+    double deflect = -fraction;
+    double E = F->el[0].E, I = F->el[0].I;
+    double linearForce = 3.0 * E * I * deflect / (F->L0 * F->L0);
+    int nSteps = 100;
+    double dFmax = linearForce / (double)nSteps;
+    double T[2][2] = {{cos(F->angle0), -sin(F->angle0)}, {sin(F->angle0), cos(F->angle0)}};
+    double Tt[2][2] = {{T[0][0], T[1][0]}, {T[0][1], T[1][1]}};
+    double local[2] = {0.0, dFmax}, global[2];
+    mat2_vec(T, local, global);
+    F->R[nd - 3] = global[0];
+    F->R[nd - 2] = global[1];
+    double gain = 1e-3, TOL = 1e-10, error;
+    int MAXIT = 10000, it = 0;
+    do {
+        fem_static(c, F);
+        double dv[2] = {F->U[nd - 3], F->U[nd - 2]}, dl[2];
+        mat2_vec(Tt, dv, dl);
+        error = deflect - dl[1] / F->L0;
+        double fc = error * gain;
+        if (fabs(fc) > fabs(dFmax)) fc = (double)((0.0 < fc) - (fc < 0.0)) * fabs(dFmax);
+        local[1] += fc;
+        mat2_vec(T, local, global);
+        F->R[nd - 3] = global[0];
+        F->R[nd - 2] = global[1];
+        it++;
+        if (it >= MAXIT) { set_err("max iterations hit! Try changing the gain...exiting"); return -1; }
+    } while (fabs(error) > TOL);
+    F->itNR = 0;
+    F->resNR = 0.0;
+    memset(F->R, 0, sizeof(double) * nd);
+    memset(F->delU, 0, sizeof(double) * nd);
+    return 0;
+}
+
 /* FEMBodyClass::predictor, FEMBody.cpp:259-289. */
 static void fem_predictor(orc_case *c, fem_t *F) {
     int nd = F->ndof;
@@ -1425,7 +1493,13 @@ orc_case *orc_create(const orc_params *p, const char *geometry) {
     for (int b = 0; b < c->nb; b++) if (c->b[b].flex == 0) c->hasFlex = 1;
     if (c->hasIBM) {
         /* ObjectsClass::initialiseObjects, Objects.cpp:941-961 */
-        if (p->initial_deflect != 0.0) { set_err("INITIAL_DEFLECT is not restated in the oracle"); orc_destroy(c); return NULL; }
+        /* ObjectsClass::initialDeflect, Objects.cpp:964-976 */
+        if (p->initial_deflect != 0.0 && c->hasFlex)
+            for (int b = 0; b < c->nb; b++)
+                if (c->b[b].flex == 0 && fem_initial_deflection(c, c->b[b].fem, p->initial_deflect)) {
+                    orc_destroy(c);
+                    return NULL;
+                }
         for (int n = 0; n < c->nm; n++) {
             if (marker_support(c, &c->m[n])) { orc_destroy(c); return NULL; }
             marker_ds(c, &c->m[n]);

@@ -110,6 +110,21 @@ def test_canopy_small(oracle, gpu_lib):
     compare_fields(oc, gc)
 
 
+def test_inverted_flag_initial_deflection(oracle, gpu_lib):
+    """INITIAL_DEFLECT (the reference's InvertedFlag example): static FEM at
+    creation, then the coupled steps, against the oracle."""
+    p, g = cfg.inverted_flag(Nx=192, Ny=96, n_el=12, Re=150.0, kb=0.3, initial_deflect=0.01)
+    oc, gc = build(oracle, p, g)
+    assert rel_err(gc.fem(0)[0], oc.fem(0)[0]) <= TOL
+    compare_markers(oc, gc)
+    oc.step(12)
+    gc.step(12)
+    assert rel_err(gc.fem(0)[0], oc.fem(0)[0]) <= TOL
+    compare_markers(oc, gc)
+    compare_fields(oc, gc)
+    assert rel_err(gc.total_force(), oc.total_force()) <= TOL
+
+
 def test_layer1_epsilon_matrix(oracle, gpu_lib):
     """life_ibm_epsilon_matrix + life_solve_lapack reproduce epsilon."""
     from life_amd import lib

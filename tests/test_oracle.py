@@ -285,3 +285,21 @@ def test_cantilever_period(oracle):
     assert 2 * np.mean(np.diff(tz)) == pytest.approx(T, rel=1e-2)
     d0 = P * L ** 3 / (3 * E * I)
     assert np.max(tip) <= d0 * 1.001 and np.min(tip) >= -d0 * 1.001
+
+
+def test_initial_deflection_reaches_target(oracle):
+    """INITIAL_DEFLECT (FEMBody.cpp:352-445): the static feedback loop stops
+    when the tip's body-frame deflection is -fraction * L0 within its 1e-10
+    tolerance; the markers follow the deflected beam."""
+    p, g = cfg.inverted_flag(Nx=256, Ny=128, n_el=20, initial_deflect=0.01)
+    oc = oracle.OracleCase(p, g)
+    U = oc.fem(0)[0]
+    nd = len(U)
+    a = math.pi  # plate angle 180 degrees
+    local_y = -math.sin(a) * U[nd - 3] + math.cos(a) * U[nd - 2]
+    assert abs(local_y / p.ref_L + 0.01) <= 1.0001e-10
+    pos = oc.markers()["pos"]
+    # free end (upstream tip) displaced by +0.01 L in y, clamped end fixed
+    assert abs(pos[-1, 1] - (0.5 * (p.Ny - 1) + 0.01 * p.ref_L)) < 1e-3 * p.ref_L
+    assert pos[0, 1] == 0.5 * (p.Ny - 1)
+    oc.close()
