@@ -74,6 +74,17 @@ def cpu_baseline(p, g, steps, threads):
     return p.Nx * p.Ny * steps / dt / 1e6
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def calibrate_copy(nbytes, reps):
     """Device-to-device copies of a known size, for scaling the rocprofv3
     FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
@@ -333,7 +344,7 @@ def main():
         v = cpu_baseline(p, geom, args.cpu_steps, threads)
         cpu = {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
                "sample": "%d full steps (lattice + coupling) of the same workload in FP64 after 1 step" %
-                         args.cpu_steps}
+                         args.cpu_steps, "cpu_model": cpu_model()}
 
     out = {
         "metric": "D2Q9 MLUPS (full step incl. IBM/FEM coupling)",

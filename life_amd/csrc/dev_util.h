@@ -35,9 +35,9 @@ constexpr int kPushBlock = 256;
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
 
-template <typename T>
+template <typename T, int BS = kPushBlock>
 struct PushStage {
-    T slot[6][kPushBlock + 2];
+    T slot[6][BS + 2];
 };
 
 // P: T* or a pointer to T in the global address space.
@@ -48,10 +48,10 @@ __device__ __forceinline__ void put(P p, T v) {
 }
 
 // fo: the nine post-collision values of site id = base + t (row i, column j).
-template <typename T, bool NT, typename DP>
+template <typename T, bool NT, int BS = kPushBlock, typename DP>
 __device__ __forceinline__ void push_store_realigned(DP __restrict__ dst, long long S, int Nx, int Ny, long long N,
                                                      long long base, int t, long long id, int i, int j,
-                                                     const T fo[9], bool valid, PushStage<T>& st) {
+                                                     const T fo[9], bool valid, PushStage<T, BS>& st) {
     const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
     const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
     if (valid) {
@@ -71,7 +71,7 @@ __device__ __forceinline__ void push_store_realigned(DP __restrict__ dst, long l
         const bool wrap = jd < 0 || jd >= Ny;
         const int to = t + cy;  // destination offset within the block
         if (valid) {
-            if (!wrap && to >= 0 && to < kPushBlock) st.slot[q][to + 1] = fo[kv[q]];
+            if (!wrap && to >= 0 && to < BS) st.slot[q][to + 1] = fo[kv[q]];
             else dst[kv[q] * S + rowv[q] + (wrap ? (jd < 0 ? Ny - 1 : 0) : jd)] = fo[kv[q]];
         }
     }
@@ -83,7 +83,7 @@ __device__ __forceinline__ void push_store_realigned(DP __restrict__ dst, long l
         for (int q = 0; q < 6; q++) {
             const int cy = kcy[q];
             const int from = t - cy;
-            const bool src_in = from >= 0 && from < kPushBlock && base + from < N;
+            const bool src_in = from >= 0 && from < BS && base + from < N;
             // the source site j - cy must not have wrapped to reach j
             const bool src_nowrap = cy > 0 ? j != 0 : j != Ny - 1;
             if (src_in && src_nowrap) {

@@ -202,31 +202,21 @@ __global__ void k_eps_far_check_batch(const CaseDev* __restrict__ cases, const C
 // registers: register slot r holds row k+r, and lane L < C holds the
 // window column j with j = L (mod C), so columns never move between lanes.
 // Step k:
-//  - column k is broadcast from its lane (v_readlane, no LDS traffic);
-//  - the pivot is the first row of maximum magnitude: the maximum over the
-//    window (rows past n hold zeros) and then the first row equal to it,
-//    which is the sequential first-maximum search for non-NaN entries;
-//  - rows k and k+p are swapped, and lane r forms the multiplier
-//    l_r = a(k+r, k) / a(k, k) (one division for the whole wavefront);
-//  - every other column gets a_ij - l_i a_kj exactly once, from the same
-//    operands as the dense algorithm, and row k of U is emitted.
+//  - the lane holding column k finds the pivot (first row of maximum
+//    magnitude; rows past n hold zeros) in its own registers, and one
+//    v_readlane makes the row index uniform;
+//  - rows k and k+p are swapped in every column;
+//  - column k goes through LDS to lanes 1..KB, lane r forms the multiplier
+//    l_r = a(k+r, k) / a(k, k) (one division for the whole wavefront), and
+//    the multipliers come back through LDS;
+//  - every other column gets a_ij - l_i a_kj once, from the same operands
+//    as the dense algorithm (rows past n get l = 0: no change), and row k
+//    of U is emitted.
 // The slots then shift by one row (row k+R enters), and column k's lane
 // takes column k+C, whose entries in rows k+1..k+KB are structural zeros.
-// Back substitution runs on one lane in increasing column order.  The
-// skipped dense operations all involve an exact zero, so the values equal
-// the dense elimination's.  The band is staged in LDS (rows of U replace
-// rows of the band as they are emitted) when it fits in lds_rows rows.
-
-// Lane `lane`'s value, as a per-lane (VGPR) copy: the identity DPP move
-// keeps the pivot logic in vector registers.
-__device__ inline double bcast64(double v, int lane) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-    const unsigned vlo = (unsigned)__builtin_amdgcn_update_dpp(0, lo, 0xE4, 0xF, 0xF, false);
-    const unsigned vhi = (unsigned)__builtin_amdgcn_update_dpp(0, hi, 0xE4, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((unsigned long long)vhi << 32) | vlo);
-}
+// The skipped dense operations all involve an exact zero, so the values
+// equal the dense elimination's.  A single wavefront is latency-bound
+// here: every step is one dependent chain (search, division, update).
 
 // Lane `lane`'s value as a wavefront-uniform (SGPR) value.
 __device__ inline double uniform64(double v, int lane) {
@@ -236,12 +226,52 @@ __device__ inline double uniform64(double v, int lane) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Back substitution on one lane in increasing column order.  Terms past n
+// are 0 * 0 (structural zeros of U times the zero-initialised x window),
+// which leave s unchanged, so rows need no bounds tests; the next row's
+// entries are loaded while the current row's chain runs.
+template <int KB>
+__device__ __forceinline__ void band_back_substitute(int first, int n, const double* __restrict__ B,
+                                                     const double* __restrict__ y, double* __restrict__ eps) {
+    constexpr int W = 3 * KB + 1;
+    double xw[2 * KB];
+#pragma unroll
+    for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
+    double u[2 * KB + 1], yi;
+    {
+        const double* Ui = B + (n - 1) * W + KB;
+#pragma unroll
+        for (int q = 0; q <= 2 * KB; q++) u[q] = Ui[q];
+        yi = y[n - 1];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double un[2 * KB + 1], yn = 0.0;
+        if (i > 0) {
+            const double* Un = B + (i - 1) * W + KB;
+#pragma unroll
+            for (int q = 0; q <= 2 * KB; q++) un[q] = Un[q];
+            yn = y[i - 1];
+        }
+        double s = yi;
+#pragma unroll
+        for (int q = 0; q < 2 * KB; q++) s = s - u[1 + q] * xw[q];
+        const double x = s / u[0];
+        eps[first + i] = x;
+#pragma unroll
+        for (int q = 2 * KB - 1; q > 0; q--) xw[q] = xw[q - 1];
+        xw[0] = x;
+#pragma unroll
+        for (int q = 0; q <= 2 * KB; q++) u[q] = un[q];
+        yi = yn;
+    }
+}
+
 // B: the band (rows of U replace it), y: the right-hand side; both either
 // in LDS or in global memory (LDS_ selects the instantiation, so that every
-// access is compiled for its own address space).
+// access is compiled for its own address space).  xb: 2R doubles of LDS.
 template <int KB, bool LDS_>
 __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __restrict__ B, double* __restrict__ y,
-                                           double* __restrict__ eps) {
+                                                    double* __restrict__ eps, double* __restrict__ xb) {
     constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
     const int L = threadIdx.x;
     auto A0 = [&](int r, int col) -> double {
@@ -255,108 +285,85 @@ __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __
         w[r] = A0(r, L);
         yw[r] = r < n ? 1.0 : 0.0;
     }
-    // Register slot (kk + r) % R holds row k + r at step k = k0 + kk: the k
-    // loop is unrolled R times so that the slots rotate with constant
-    // indices instead of moving.
     int pl = 0;   // lane holding column k
     int off = L;  // this lane holds column k + off
-    for (int k0 = 0; k0 < n; k0 += R) {
+    for (int k = 0; k < n; k++) {
+        // row k+R enters this lane's column after the step (band offset
+        // q = column - row + KB, always inside the stored band here)
+        const int cn = L == pl ? C : off;
+        const double incoming = (L < C && k + R < n && k + cn < n) ? B[(k + R) * W + (cn - 1)] : 0.0;
+        double best = fabs(w[0]);
+        int pr = 0;
 #pragma unroll
-        for (int kk = 0; kk < R; kk++) {
-            const int k = k0 + kk;
-            if (k < n) {
-                auto S = [&](int r) -> int { return (kk + r) % R; };
-                // row k+R enters this lane's column after the step (band
-                // offset q = column - row + KB, always inside the stored band)
-                const int cn = L == pl ? C : off;
-                const double incoming = (L < C && k + R < n && k + cn < n) ? B[(k + R) * W + (cn - 1)] : 0.0;
-                const int rmax = min(KB, n - 1 - k);
-                // first row of maximum magnitude, found by the lane that holds
-                // column k in its own registers
-                double best = fabs(w[S(0)]);
-                int pr = 0;
-#pragma unroll
-                for (int r = 1; r < R; r++)
-                    if (r <= rmax) {
-                        const double a = fabs(w[S(r)]);
-                        if (a > best) {
-                            best = a;
-                            pr = r;
-                        }
-                    }
-                const int p = __builtin_amdgcn_readlane(pr, pl);
-                if (__builtin_amdgcn_readlane(best != 0.0 ? 1 : 0, pl)) {
-                    if (p != 0) {
-                        const double tw = w[S(0)], ty = yw[S(0)];
-#pragma unroll
-                        for (int r = 1; r < R; r++)
-                            if (r == p) {
-                                w[S(0)] = w[S(r)];
-                                w[S(r)] = tw;
-                                yw[S(0)] = yw[S(r)];
-                                yw[S(r)] = ty;
-                            }
-                    }
-                    // multipliers l_r = a(k+r, k) / a(k, k): column k goes to
-                    // lanes 1..KB (lane r gets row r), one division serves all
-                    const double piv = uniform64(w[S(0)], pl);
-                    double num = 0.0;
-#pragma unroll
-                    for (int r = 1; r < R; r++) num = L == r ? uniform64(w[S(r)], pl) : num;
-                    const double lv = num / piv;
-#pragma unroll
-                    for (int r = 1; r < R; r++)
-                        if (r <= rmax) {
-                            const double l = uniform64(lv, r);
-                            const double upd = w[S(r)] - l * w[S(0)];
-                            w[S(r)] = L == pl ? 0.0 : upd;
-                            yw[S(r)] = yw[S(r)] - l * yw[S(0)];
-                        }
-                }
-                // row k of U (in place of row k of the band) and the final y_k
-                if (L < C && k + off < n) B[k * W + (off + KB)] = w[S(0)];
-                if (L == 0) y[k] = yw[S(0)];
-                // row k leaves, row k+R takes its slot; column k's lane takes
-                // column k+C, whose entries in rows k+1..k+KB are zeros
-                w[S(0)] = incoming;
-                yw[S(0)] = k + R < n ? 1.0 : 0.0;
-                off = L == pl ? C - 1 : off - 1;
-                pl = pl == C - 1 ? 0 : pl + 1;
-            }
+        for (int r = 1; r < R; r++) {
+            const double a = fabs(w[r]);
+            const bool gt = a > best;
+            best = gt ? a : best;
+            pr = gt ? r : pr;
         }
+        const int p = __builtin_amdgcn_readlane(pr, pl);
+        if (p != 0) {
+            const double tw = w[0], ty = yw[0];
+#pragma unroll
+            for (int r = 1; r < R; r++)
+                if (r == p) {
+                    w[0] = w[r];
+                    w[r] = tw;
+                    yw[0] = yw[r];
+                    yw[r] = ty;
+                }
+        }
+        if (L == pl) {
+#pragma unroll
+            for (int r = 0; r < R; r++) xb[r] = w[r];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const double piv0 = xb[0];
+        const double piv = piv0 != 0.0 ? piv0 : 1.0;  // zero column: all multipliers 0
+        const double num = (L >= 1 && L < R) ? xb[L] : 0.0;
+        if (L >= 1 && L < R) xb[R + L] = num / piv;
+        __builtin_amdgcn_wave_barrier();
+        double l[R];
+#pragma unroll
+        for (int r = 1; r < R; r++) l[r] = xb[R + r];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            const double upd = w[r] - l[r] * w[0];
+            w[r] = L == pl ? 0.0 : upd;
+            yw[r] = yw[r] - l[r] * yw[0];
+        }
+        // row k of U (in place of row k of the band) and the final y_k
+        if (L < C && k + off < n) B[k * W + (off + KB)] = w[0];
+        if (L == 0) y[k] = yw[0];
+        // drop row k, take in row k+R; column k's lane takes column k+C
+#pragma unroll
+        for (int r = 0; r < R - 1; r++) {
+            w[r] = w[r + 1];
+            yw[r] = yw[r + 1];
+        }
+        w[R - 1] = incoming;
+        yw[R - 1] = k + R < n ? 1.0 : 0.0;
+        off = L == pl ? C - 1 : off - 1;
+        pl = pl == C - 1 ? 0 : pl + 1;
+        __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    if (L == 0) {
-        double xw[2 * KB];
-#pragma unroll
-        for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
-        for (int i = n - 1; i >= 0; i--) {
-            const double* Ui = B + i * W + KB;
-            double s = y[i];
-#pragma unroll
-            for (int q = 0; q < 2 * KB; q++)
-                if (i + 1 + q < n) s = s - Ui[1 + q] * xw[q];
-            const double x = s / Ui[0];
-            eps[first + i] = x;
-#pragma unroll
-            for (int q = 2 * KB - 1; q > 0; q--) xw[q] = xw[q - 1];
-            xw[0] = x;
-        }
-    }
+    if (L == 0) band_back_substitute<KB>(first, n, B, y, eps);
 }
 
 template <int KB>
 __device__ __forceinline__ void eps_band_solve_body(int first, int n, double* __restrict__ G, double* __restrict__ ywork,
-                                           double* __restrict__ eps, int lds_rows, double* sm) {
+                                                    double* __restrict__ eps, int lds_rows, double* sm,
+                                                    double* xb) {
     constexpr int W = 3 * KB + 1;
     if (n <= lds_rows) {
         const int nw = n * W;
 #pragma unroll 8
         for (int t = threadIdx.x; t < nw; t += 64) sm[t] = G[t];
         __syncthreads();
-        eps_band_solve_impl<KB, true>(first, n, sm, sm + (long long)nw, eps);
+        eps_band_solve_impl<KB, true>(first, n, sm, sm + (long long)nw, eps, xb);
     } else {
-        eps_band_solve_impl<KB, false>(first, n, G, ywork + first, eps);
+        eps_band_solve_impl<KB, false>(first, n, G, ywork + first, eps, xb);
     }
 }
 
@@ -367,20 +374,22 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
                                                        double* __restrict__ ywork, double* __restrict__ eps,
                                                        int lds_rows) {
     extern __shared__ double sm[];
+    __shared__ double xb[2 * (KB + 1)];
     const int sys = blockIdx.x;
     if (outside[sys]) return;
-    eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm);
+    eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm, xb);
 }
 
 template <int KB>
 __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __restrict__ cases,
                                                              const CaseSys* __restrict__ sys, int lds_rows) {
     extern __shared__ double sm[];
+    __shared__ double xb[2 * (KB + 1)];
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
     if (c.eps.outside[q.k]) return;
     eps_band_solve_body<KB>(c.eps.first[q.k], c.eps.n[q.k], c.eps.band + c.eps.off[q.k], c.eps.y, c.m.eps, lds_rows,
-                            sm);
+                            sm, xb);
 }
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at

@@ -132,9 +132,14 @@ struct CaseDev {
     EpsDev eps;
 };
 
+// Sites per block of the batched stream-collide: FP32 uses the block size
+// measured best for the single-case kernel, FP64 the LDS kernel's 256.
+constexpr int kServeBlock32 = 512;
+constexpr int kServeBlock64 = 256;
+
 struct CaseBlock {
     int c;    // table index
-    int blk;  // 256-site block of that case
+    int blk;  // block of that case (kServeBlock32 or kServeBlock64 sites)
 };
 
 struct CaseSys {

@@ -74,10 +74,11 @@ __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, f
 }
 
 // Bulk stream-collide body of one block of kPushBlock sites from `base`.
-// SP/DP: float pointers, generic or in the global address space.
-template <bool CM, typename SP, typename DP>
+// SP/DP: float pointers, generic or in the global address space.  BS:
+// sites per block; NTS: streaming hint on the stores.
+template <bool CM, typename SP, typename DP, int BS = kPushBlock, bool NTS = true>
 __device__ __forceinline__ void push32_lds_block(SP __restrict__ src, DP __restrict__ dst, const GridScalars& g,
-                                                 long long base, int t, PushStage<float>& st) {
+                                                 long long base, int t, PushStage<float, BS>& st) {
     const long long id = base + t;
     const bool valid = id < g.N;
     const long long idc = valid ? id : 0;
@@ -91,28 +92,28 @@ __device__ __forceinline__ void push32_lds_block(SP __restrict__ src, DP __restr
     const float Fx = (float)g.Fx, Fy = (float)g.Fy;
     moments32(gs, Fx, Fy, drho, ux, uy);
     collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
-    push_store_realigned<float, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, go, valid, st);
+    push_store_realigned<float, NTS, BS>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, go, valid, st);
 }
 
 // Bulk stream-collide with LDS-realigned stores (see push_store_realigned).
-template <bool CM, bool XCD = true>
-__global__ __launch_bounds__(kPushBlock) void k_push32_lds(const float* __restrict__ src, float* __restrict__ dst,
-                                                           GridScalars g, unsigned nblocks) {
-    __shared__ PushStage<float> st;
-    const long long base = (long long)(XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x) * kPushBlock;
-    push32_lds_block<CM>(src, dst, g, base, threadIdx.x, st);
+template <bool CM, bool XCD = true, int BS = kPushBlock, bool NTS = true>
+__global__ __launch_bounds__(BS) void k_push32_lds(const float* __restrict__ src, float* __restrict__ dst,
+                                                   GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<float, BS> st;
+    const long long base = (long long)(XCD ? xcd_block(blockIdx.x, nblocks) : blockIdx.x) * BS;
+    push32_lds_block<CM, const float*, float*, BS, NTS>(src, dst, g, base, threadIdx.x, st);
 }
 
 // Serving: one launch over the blocks of every admitted case of one
 // collision operator.  A case's first step after admission takes its
 // rho/u/force_ibm as stored (the explicit path of k_push32).
 template <bool CM>
-__global__ __launch_bounds__(kPushBlock) void k_push32_batch(const CaseDev* __restrict__ cases,
-                                                             const CaseBlock* __restrict__ blocks) {
-    __shared__ PushStage<float> st;
+__global__ __launch_bounds__(kServeBlock32) void k_push32_batch(const CaseDev* __restrict__ cases,
+                                                                const CaseBlock* __restrict__ blocks) {
+    __shared__ PushStage<float, kServeBlock32> st;
     const CaseBlock b = blocks[blockIdx.x];
     const CaseDev& c = cases[b.c];
-    const long long base = (long long)b.blk * kPushBlock;
+    const long long base = (long long)b.blk * kServeBlock32;
     if (c.explicit_state) {
         const long long id = base + threadIdx.x;
         if (id < c.g.N)
@@ -121,7 +122,9 @@ __global__ __launch_bounds__(kPushBlock) void k_push32_batch(const CaseDev* __re
     } else {
         // table pointers are global memory: typed so, the bulk loads and
         // stores are global (not flat) instructions
-        push32_lds_block<CM>((gptr<const float>)c.src, (gptr<float>)c.dst, c.g, base, threadIdx.x, st);
+        push32_lds_block<CM, gptr<const float>, gptr<float>, kServeBlock32>((gptr<const float>)c.src,
+                                                                           (gptr<float>)c.dst, c.g, base,
+                                                                           threadIdx.x, st);
     }
 }
 
@@ -232,15 +235,20 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
 }
 
 // Launch configuration of the bulk stream-collide kernel, read once from
-// the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 (kernel),
-// LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32), LIFE_PUSH32_XCD
-// = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256 (block size).
+// the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 | lds
+// (kernel), LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32),
+// LIFE_PUSH32_XCD = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256
+// (block size of k_push32), LIFE_PUSH32_LDSBS = 256 | 512 | 1024 (block size
+// of k_push32_lds; 512 measured best on MI355X), LIFE_PUSH32_LDSNT = 0 | 1
+// (streaming stores in k_push32_lds).
 struct PushConfig {
     bool x4 = false;
     bool lds = true;
     int nt = 3;
     bool xcd = true;
     int bs = 256;
+    int lds_bs = 512;
+    bool lds_nt = true;
     PushConfig() {
         if (const char* e = std::getenv("LIFE_PUSH32")) {
             x4 = std::strcmp(e, "x4") == 0;
@@ -250,6 +258,11 @@ struct PushConfig {
         if (const char* e = std::getenv("LIFE_PUSH32_XCD")) xcd = std::atoi(e) != 0;
         if (const char* e = std::getenv("LIFE_PUSH32_BS")) bs = std::atoi(e);
         if (bs != 64 && bs != 128 && bs != 256) bs = 256;
+        if (const char* e = std::getenv("LIFE_PUSH32_LDSBS")) {
+            const int v = std::atoi(e);
+            lds_bs = v == 256 || v == 1024 ? v : 512;
+        }
+        if (const char* e = std::getenv("LIFE_PUSH32_LDSNT")) lds_nt = std::atoi(e) != 0;
     }
 };
 
@@ -269,9 +282,19 @@ template <bool CM>
 static void launch_push32_bulk(const float* src, float* dst, const GridScalars& g, hipStream_t s) {
     const PushConfig& c = push_config();
     if (c.lds) {
-        const unsigned nb = (unsigned)((g.N + kPushBlock - 1) / kPushBlock);
-        if (c.xcd) k_push32_lds<CM, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
-        else k_push32_lds<CM, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        const unsigned nb = (unsigned)((g.N + c.lds_bs - 1) / c.lds_bs);
+        if (c.lds_bs == 512) {
+            if (c.lds_nt) k_push32_lds<CM, true, 512, true><<<nb, 512, 0, s>>>(src, dst, g, nb);
+            else k_push32_lds<CM, true, 512, false><<<nb, 512, 0, s>>>(src, dst, g, nb);
+        } else if (c.lds_bs == 1024) {
+            k_push32_lds<CM, true, 1024, true><<<nb, 1024, 0, s>>>(src, dst, g, nb);
+        } else if (!c.lds_nt) {
+            k_push32_lds<CM, true, kPushBlock, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        } else if (c.xcd) {
+            k_push32_lds<CM, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        } else {
+            k_push32_lds<CM, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        }
         return;
     }
     if (c.x4 && g.Ny % 4 == 0) {
@@ -314,8 +337,8 @@ void launch_support_push32(const float* src, float* dst, const GridScalars& g, c
 
 void launch_push32_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    if (cm) k_push32_batch<true><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
-    else k_push32_batch<false><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+    if (cm) k_push32_batch<true><<<(unsigned)nblocks, kServeBlock32, 0, s>>>(cases, blocks);
+    else k_push32_batch<false><<<(unsigned)nblocks, kServeBlock32, 0, s>>>(cases, blocks);
 }
 
 void launch_support_push32_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s) {

@@ -106,12 +106,12 @@ __device__ __forceinline__ void push_site64(double* __restrict__ dst, const Grid
 // Serving: one launch over the blocks of every admitted case of one
 // collision operator; a case's first step takes rho/u/force_ibm as stored.
 template <bool CM>
-__global__ __launch_bounds__(kPushBlock) void k_push64_batch(const CaseDev* __restrict__ cases,
+__global__ __launch_bounds__(kServeBlock64) void k_push64_batch(const CaseDev* __restrict__ cases,
                                                              const CaseBlock* __restrict__ blocks) {
     __shared__ PushStage<double> st;
     const CaseBlock b = blocks[blockIdx.x];
     const CaseDev& c = cases[b.c];
-    const long long base = (long long)b.blk * kPushBlock;
+    const long long base = (long long)b.blk * kServeBlock64;
     const double* src = static_cast<const double*>(c.src);
     double* dst = static_cast<double*>(c.dst);
     if (c.explicit_state) {
@@ -195,8 +195,8 @@ void launch_support_push64(const double* src, double* dst, const GridScalars& g,
 
 void launch_push64_batch(bool cm, const CaseDev* cases, const CaseBlock* blocks, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    if (cm) k_push64_batch<true><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
-    else k_push64_batch<false><<<(unsigned)nblocks, kPushBlock, 0, s>>>(cases, blocks);
+    if (cm) k_push64_batch<true><<<(unsigned)nblocks, kServeBlock64, 0, s>>>(cases, blocks);
+    else k_push64_batch<false><<<(unsigned)nblocks, kServeBlock64, 0, s>>>(cases, blocks);
 }
 
 void launch_support_push64_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_sites, hipStream_t s) {

@@ -237,7 +237,8 @@ void rebuild(life_server* s) {
         Slot& sl = s->slots[q];
         life_grid* g = sl.c->g;
         const int cm = g->g.cm ? 1 : 0;
-        const long long nb = (g->g.N + 255) / 256;
+        const int bs = s->cfg.precision == LIFE_FP64 ? kServeBlock64 : kServeBlock32;
+        const long long nb = (g->g.N + bs - 1) / bs;
         for (long long b = 0; b < nb; b++) P.blocks[cm].push_back(CaseBlock{q, (int)b});
         if (g->convective) P.conv.push_back(q);
         if (g->nbc > 0) {
