@@ -259,7 +259,19 @@ void recompute_object_vals(life_case* c) {
 }
 
 // ObjectsClass::objectKernel, Objects.cpp:26-60.
-void object_kernel(life_case* c) {
+// Pending early stream-collide of the next step (launch point, events).
+struct EarlyBulk {
+    bool want = false;
+    hipEvent_t begin = nullptr, end = nullptr;
+};
+
+void launch_pending(life_case* c, EarlyBulk& eb) {
+    if (!eb.want) return;
+    lbm_bulk_early(c->g, eb.begin, eb.end);
+    eb.want = false;
+}
+
+void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
     c->subIt = 0;
     c->forces_on_host = false;
     PhaseTimes& T = *c->timing;
@@ -267,6 +279,9 @@ void object_kernel(life_case* c) {
         if (c->hasFlex) recompute_object_vals(c);
         T.lap(c->g, 1);
         ibm_interpolate(c->g);
+        // mode 2: the next step's stream-collide starts once the first
+        // pass of the coupling kernels is queued, beside the host FEM
+        if (mode == 2) launch_pending(c, eb);
         if (!c->hasFlex) break;
         download_forces(c);
         T.lap(c->g, 2);
@@ -289,13 +304,39 @@ void object_kernel(life_case* c) {
     if (c->subIt == kMaxSubIt) throw std::runtime_error("Subiteration scheme hit 20 iterations");
 }
 
-void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end) {
+// The bulk stream-collide of step t+1 reads only the populations after
+// step t and the uniform force, so it is launched on the grid's second
+// stream before the coupling of step t and overlaps it (disabled with
+// LIFE_OVERLAP=0).  Support sites and boundaries of step t+1 still follow
+// the spread of step t, as in the reference order.
+// LIFE_OVERLAP = 0 (off), 1 (launch right after this step's lattice
+// kernels, default; measured fastest) or 2 (after the first interpolation
+// of the coupling).
+int overlap_mode() {
+    static const int mode = [] {
+        const char* e = std::getenv("LIFE_OVERLAP");
+        if (!e) return 1;
+        const int v = std::atoi(e);
+        return v >= 0 && v <= 2 ? v : 1;
+    }();
+    return mode;
+}
+
+// One main-loop step; next_begin/next_end (may be null) are recorded
+// around the next step's stream-collide when it is launched early
+// (more = a next step follows in this call).
+void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, bool more, hipEvent_t next_begin,
+               hipEvent_t next_end) {
     PhaseTimes& T = *c->timing;
     T.start(c->g);
     c->t++;
     lbm_step(c->g, c->t, ev_begin, ev_end);
     T.lap(c->g, 0);
-    if (c->hasIBM) object_kernel(c);
+    const int mode = T.on ? 0 : overlap_mode();
+    EarlyBulk eb{more && mode != 0, next_begin, next_end};
+    if (mode == 1 || !c->hasIBM) launch_pending(c, eb);
+    if (c->hasIBM) object_kernel(c, eb, mode);
+    launch_pending(c, eb);
     T.t[5] += 1;
 }
 
@@ -374,7 +415,7 @@ int life_case_destroy(life_case* c) {
 int life_case_step(life_case* c, int nsteps) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(c->g->device));
-        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr);
+        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr, s + 1 < nsteps, nullptr, nullptr);
         LIFE_HIP(hipStreamSynchronize(c->g->stream));
         if (c->hasIBM) check_flags(c->g);
     });
@@ -393,7 +434,15 @@ int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_
             c->kev.push_back(e);
         }
         LIFE_HIP(hipEventRecord(g->ev_a, g->stream));
-        for (int s = 0; s < nsteps; s++) case_step(c, c->kev[2 * s], c->kev[2 * s + 1]);
+        // a step's stream-collide events are recorded where it is launched:
+        // in the step itself, or during the previous step when launched early
+        for (int s = 0; s < nsteps; s++) {
+            const bool early = g->bulk_pending;
+            hipEvent_t nb = s + 1 < nsteps ? c->kev[2 * (s + 1)] : nullptr;
+            hipEvent_t ne = s + 1 < nsteps ? c->kev[2 * (s + 1) + 1] : nullptr;
+            case_step(c, early ? nullptr : c->kev[2 * s], early ? nullptr : c->kev[2 * s + 1], s + 1 < nsteps, nb,
+                      ne);
+        }
         LIFE_HIP(hipEventRecord(g->ev_b, g->stream));
         LIFE_HIP(hipEventSynchronize(g->ev_b));
         float ms = 0.0f;

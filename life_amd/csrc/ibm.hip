@@ -11,6 +11,12 @@
 
 namespace life {
 
+// The single-case coupling kernels run beside the next step's bulk
+// stream-collide (launched early on another stream): their waves take the
+// highest issue priority on a SIMD they share with stream-collide waves,
+// since they are the critical path of the step.
+__device__ __forceinline__ void coupling_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // IBMNodeClass::findSupport (IBMNode.cpp:139-179): lattice sites within 1.5
 // spacings of the marker in both directions, scanned i-major over the 5x5
 // block around the nearest site, kept in scan order (at most 9).
@@ -47,6 +53,7 @@ __device__ inline void find_support_one(const MarkerDev& m, const GridScalars& g
 }
 
 __global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
+    coupling_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < m.n) find_support_one(m, g, flexible_only, k);
 }
@@ -88,6 +95,7 @@ __device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g,
 }
 
 __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
+    coupling_priority();
     extern __shared__ double spos[];
     compute_ds_body(m, g, flexible_only, blockIdx.x, spos);
 }
@@ -152,6 +160,7 @@ __device__ inline void eps_band_entry(const MarkerDev& m, const GridScalars& g, 
 __global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
                                    const int* __restrict__ sys_n, int KB, double* __restrict__ band,
                                    const long long* __restrict__ band_off) {
+    coupling_priority();
     const int sys = blockIdx.y;
     eps_band_entry(m, g, sys_first[sys], sys_n[sys], KB, band + band_off[sys],
                    (long long)blockIdx.x * blockDim.x + threadIdx.x);
@@ -173,6 +182,7 @@ __device__ inline void eps_far_pair(const MarkerDev& m, const GridScalars& g, in
 
 __global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
                                 const int* __restrict__ sys_n, int KB, int* __restrict__ outside) {
+    coupling_priority();
     const int sys = blockIdx.y;
     eps_far_pair(m, g, sys_first[sys], sys_n[sys], KB, outside + sys,
                  (long long)blockIdx.x * blockDim.x + threadIdx.x);
@@ -226,43 +236,49 @@ __device__ inline double uniform64(double v, int lane) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Back substitution on one lane in increasing column order.  Terms past n
-// are 0 * 0 (structural zeros of U times the zero-initialised x window),
-// which leave s unchanged, so rows need no bounds tests; the next row's
-// entries are loaded while the current row's chain runs.
+// Back substitution on one lane in increasing column order, two rows per
+// iteration (no register shuffling between them).  Terms past n are 0 * 0
+// (structural zeros of U times the zero-initialised x window), which
+// leave s unchanged, so rows need no bounds tests.
 template <int KB>
 __device__ __forceinline__ void band_back_substitute(int first, int n, const double* __restrict__ B,
                                                      const double* __restrict__ y, double* __restrict__ eps) {
     constexpr int W = 3 * KB + 1;
-    double xw[2 * KB];
+    double xa[2 * KB + 1];  // xa[q] = x[i+1+q] for the row pair (i, i-1)
 #pragma unroll
-    for (int q = 0; q < 2 * KB; q++) xw[q] = 0.0;
-    double u[2 * KB + 1], yi;
-    {
-        const double* Ui = B + (n - 1) * W + KB;
+    for (int q = 0; q <= 2 * KB; q++) xa[q] = 0.0;
+    int i = n - 1;
+    for (; i >= 1; i -= 2) {
+        const double* U1 = B + i * W + KB;
+        const double* U0 = B + (i - 1) * W + KB;
+        double u1[2 * KB + 1], u0[2 * KB + 1];
 #pragma unroll
-        for (int q = 0; q <= 2 * KB; q++) u[q] = Ui[q];
-        yi = y[n - 1];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-        double un[2 * KB + 1], yn = 0.0;
-        if (i > 0) {
-            const double* Un = B + (i - 1) * W + KB;
-#pragma unroll
-            for (int q = 0; q <= 2 * KB; q++) un[q] = Un[q];
-            yn = y[i - 1];
+        for (int q = 0; q <= 2 * KB; q++) {
+            u1[q] = U1[q];
+            u0[q] = U0[q];
         }
-        double s = yi;
+        double s = y[i];
 #pragma unroll
-        for (int q = 0; q < 2 * KB; q++) s = s - u[1 + q] * xw[q];
-        const double x = s / u[0];
-        eps[first + i] = x;
+        for (int q = 0; q < 2 * KB; q++) s = s - u1[1 + q] * xa[q];
+        const double x1 = s / u1[0];
+        eps[first + i] = x1;
+        double t = y[i - 1];
+        t = t - u0[1] * x1;
 #pragma unroll
-        for (int q = 2 * KB - 1; q > 0; q--) xw[q] = xw[q - 1];
-        xw[0] = x;
+        for (int q = 1; q < 2 * KB; q++) t = t - u0[1 + q] * xa[q - 1];
+        const double x0 = t / u0[0];
+        eps[first + i - 1] = x0;
 #pragma unroll
-        for (int q = 0; q <= 2 * KB; q++) u[q] = un[q];
-        yi = yn;
+        for (int q = 2 * KB; q > 1; q--) xa[q] = xa[q - 2];
+        xa[1] = x1;
+        xa[0] = x0;
+    }
+    if (i == 0) {
+        const double* U = B + KB;
+        double s = y[0];
+#pragma unroll
+        for (int q = 0; q < 2 * KB; q++) s = s - U[1 + q] * xa[q];
+        eps[first] = s / U[0];
     }
 }
 
@@ -376,6 +392,7 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
     const int sys = blockIdx.x;
+    coupling_priority();
     if (outside[sys]) return;
     eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm, xb);
 }
@@ -416,6 +433,7 @@ __device__ inline void interpolate_one(const T* __restrict__ f, const MarkerDev&
 
 template <typename T>
 __global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
+    coupling_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < m.n) interpolate_one(f, m, g, k);
 }
@@ -508,14 +526,17 @@ __global__ void k_spread_clear(double* __restrict__ fibm, long long S, const int
 __global__ void k_spread_reset(int* uniq_n) { *uniq_n = 0; }
 
 __global__ void k_spread_count(MarkerDev m, SpreadScratch sc) {
+    coupling_priority();
     spread_count_one(m, sc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_spread_scatter(MarkerDev m, SpreadScratch sc) {
+    coupling_priority();
     spread_scatter_one(m, sc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restrict__ fibm, long long S) {
+    coupling_priority();
     spread_reduce_one(m, sc, fibm, S, blockIdx.x * blockDim.x + threadIdx.x);
 }
 

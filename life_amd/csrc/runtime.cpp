@@ -107,28 +107,56 @@ int guarded(F&& f) {
 
 }  // namespace
 
+void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    if (g->explicit_state) throw std::runtime_error("early stream-collide needs the populations' own moments");
+    const int src = g->cur, dst = 1 - g->cur;
+    LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
+    LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_ready, 0));
+    hipStream_t s = g->stream2;
+    if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+    if (g->prec == kFP64)
+        launch_push64(static_cast<const double*>(g->f[src]), static_cast<double*>(g->f[dst]), g->g, g->rho0, g->u0,
+                      g->fibm, false, s);
+    else
+        launch_push32(static_cast<const float*>(g->f[src]), static_cast<float*>(g->f[dst]), g->g, g->rho0, g->u0,
+                      g->fibm, false, s);
+    if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+    LIFE_HIP(hipGetLastError());
+    LIFE_HIP(hipEventRecord(g->ev_bulk, s));
+    g->bulk_pending = true;
+}
+
 void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const double ramp = ramp_coefficient(g->p, g->cs.Dt, t);
     const int src = g->cur, dst = 1 - g->cur;
     BCArgs a{g->bc, g->nbc, g->u_in, g->rho_in, g->delU, ramp};
     const bool ex = g->explicit_state;
     hipStream_t s = g->stream;
+    const bool early = g->bulk_pending;
+    if (early) {
+        LIFE_HIP(hipStreamWaitEvent(s, g->ev_bulk, 0));
+        g->bulk_pending = false;
+    }
     if (g->prec == kFP64) {
         const double* fs = static_cast<const double*>(g->f[src]);
         double* fd = static_cast<double*>(g->f[dst]);
         if (g->convective) launch_convective_speed<double>(fs, g->g, g->fibm, g->u0, ex, g->ucol, g->delU, true, s);
-        if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
-        launch_push64(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
-        if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        if (!early) {
+            if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+            launch_push64(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
+            if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        }
         if (!ex && g->md.n > 0) launch_support_push64(fs, fd, g->g, g->fibm, g->sc.uniq, g->sc.uniq_n, g->sc.cap, s);
         launch_bc<double>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
     } else {
         const float* fs = static_cast<const float*>(g->f[src]);
         float* fd = static_cast<float*>(g->f[dst]);
         if (g->convective) launch_convective_speed<float>(fs, g->g, g->fibm, g->u0, ex, g->ucol, g->delU, false, s);
-        if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
-        launch_push32(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
-        if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        if (!early) {
+            if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+            launch_push32(fs, fd, g->g, g->rho0, g->u0, g->fibm, ex, s);
+            if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
+        }
         if (!ex && g->md.n > 0) launch_support_push32(fs, fd, g->g, g->fibm, g->sc.uniq, g->sc.uniq_n, g->sc.cap, s);
         launch_bc<float>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
     }
@@ -273,6 +301,9 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         LIFE_HIP(hipEventCreate(&g->ev_b));
         LIFE_HIP(hipEventCreate(&g->ev_k0));
         LIFE_HIP(hipEventCreate(&g->ev_k1));
+        LIFE_HIP(hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking));
+        LIFE_HIP(hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming));
+        LIFE_HIP(hipEventCreateWithFlags(&g->ev_bulk, hipEventDisableTiming));
         g->p = *p;
         g->prec = precision;
         g->cs = derive_scalars(*p);
@@ -399,8 +430,10 @@ int life_grid_destroy(life_grid* g) {
                         (void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
                         (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
-        for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1})
+        if (g->stream2) (void)hipStreamSynchronize(g->stream2);
+        for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk})
             if (e) (void)hipEventDestroy(e);
+        if (g->stream2) (void)hipStreamDestroy(g->stream2);
         if (g->stream) (void)hipStreamDestroy(g->stream);
         delete g;
     });

@@ -93,6 +93,10 @@ struct life_grid {
     std::vector<int> eps_first_h, eps_n_h;
     // timing
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    // early bulk stream-collide of the next step on a second stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_ready = nullptr, ev_bulk = nullptr;
+    bool bulk_pending = false;
 
     void* f_cur() { return f[cur]; }
 };
@@ -101,6 +105,14 @@ namespace life {
 // Enqueue one lbmKernel on the grid stream; if ev_begin/ev_end are given,
 // they are recorded around the fused stream-collide launch.
 void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end);
+// The same step in two parts.  lbm_bulk_early launches the bulk
+// stream-collide of the next step on the second stream as soon as the
+// populations of the current step are final (it reads only f and the
+// uniform force); the coupling of the current step can run meanwhile.
+// lbm_step then finishes that step (outlet speed, support sites,
+// boundaries) after the bulk kernel.  Not for the first step after
+// creation or upload (explicit rho/u).
+void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end);
 void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);
