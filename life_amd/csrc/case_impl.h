@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,29 @@
 #include "runtime.h"
 
 namespace life {
+
+// Page-locked host memory for arrays that the single-case driver copies to
+// and from the device every step: the copies are then DMA transfers that
+// do not block the host, so the host must not write such an array while a
+// copy out of it may still be queued.
+template <typename T>
+struct PinnedAllocator {
+    using value_type = T;
+    PinnedAllocator() = default;
+    template <typename U>
+    PinnedAllocator(const PinnedAllocator<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    template <typename U>
+    bool operator==(const PinnedAllocator<U>&) const { return true; }
+    template <typename U>
+    bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+using pinned_vector = std::vector<double, PinnedAllocator<double>>;
 
 struct Body {
     int ID;
@@ -63,7 +87,13 @@ struct life_case {
     std::vector<life::Body> bodies;
     std::vector<life::Body*> flex_bodies;  // flexible bodies in body order
     int nm = 0;
-    std::vector<double> pos, vel, force, eps, ds;
+    life::pinned_vector pos, vel, force, eps, ds;
+    // recorded after the last upload of pos/vel in a step; waited on before
+    // the host next writes them
+    hipEvent_t ev_upload = nullptr;
+    // banded epsilon systems were launched; their out-of-band flags are
+    // read at the next synchronisation (the force download)
+    bool eps_check_pending = false;
     std::vector<int> mbody;
     bool hasIBM = false, hasFlex = false;
     int subIt = 0, simDOFs = 0;

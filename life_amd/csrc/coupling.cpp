@@ -176,8 +176,20 @@ void compute_epsilon(life_case* c, bool initial) {
         for (auto& b : c->bodies)
             if (b.n > 0) solve(b.first, b.n);
     } else {
-        for (int k : ibm_epsilon_band(c->g)) solve(c->g->eps_first_h[k], c->g->eps_n_h[k]);
+        // systems found not to be banded are solved when the forces are
+        // downloaded (finish_epsilon); the interpolation in between does
+        // not read epsilon
+        ibm_epsilon_band_launch(c->g);
+        c->eps_check_pending = true;
     }
+}
+
+// Dense solves of the systems that the banded path left (after a
+// synchronisation of the grid stream).
+void finish_epsilon(life_case* c) {
+    if (!c->eps_check_pending) return;
+    c->eps_check_pending = false;
+    for (int k : ibm_epsilon_band_dense(c->g)) case_solve_eps_dense(c, c->g->eps_first_h[k], c->g->eps_n_h[k]);
 }
 
 // Marker forces (and epsilon, which the FEM load vector uses) to the host.
@@ -186,6 +198,7 @@ void download_forces(life_case* c) {
                             c->g->stream));
     LIFE_HIP(hipMemcpyAsync(c->eps.data(), c->g->md.eps, sizeof(double) * c->nm, hipMemcpyDeviceToHost, c->g->stream));
     LIFE_HIP(hipStreamSynchronize(c->g->stream));
+    finish_epsilon(c);
 }
 
 // Run fn(body) for every flexible body.  Bodies are independent (disjoint
@@ -250,6 +263,8 @@ namespace {
 
 // ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
 void recompute_object_vals(life_case* c) {
+    // the host rewrites pos/vel: the previous step's upload must be done
+    LIFE_HIP(hipEventSynchronize(c->ev_upload));
     case_recompute_host(c);
     life_grid* g = c->g;
     LIFE_HIP(hipMemcpyAsync(g->md.pos, c->pos.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice, g->stream));
@@ -299,6 +314,7 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
                                 c->g->stream));
         LIFE_HIP(hipMemcpyAsync(c->g->md.vel, c->vel.data(), sizeof(double) * 2 * c->nm, hipMemcpyHostToDevice,
                                 c->g->stream));
+        LIFE_HIP(hipEventRecord(c->ev_upload, c->g->stream));
         c->forces_on_host = true;
     }
     if (c->subIt == kMaxSubIt) throw std::runtime_error("Subiteration scheme hit 20 iterations");
@@ -367,6 +383,7 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
         if (p->initial_deflect != 0.0)
             for (Body* b : c->flex_bodies) b->fem->set_initial_deflection(p->initial_deflect, c->view(*b));
         if (life_grid_create(p, precision, device, &c->g)) throw std::runtime_error(life_last_error());
+        LIFE_HIP(hipEventCreateWithFlags(&c->ev_upload, hipEventDisableTiming));
         if (c->hasIBM) {
             std::vector<int> flex(c->bodies.size());
             for (size_t b = 0; b < c->bodies.size(); b++) flex[b] = c->bodies[b].flex;
@@ -406,6 +423,7 @@ int life_case_destroy(life_case* c) {
     if (c->g) {
         (void)hipSetDevice(c->g->device);
         for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+        if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
         rc = life_grid_destroy(c->g);
     }
     delete c;

@@ -10,9 +10,12 @@ namespace life {
 
 namespace {
 
-// Utils::shiftAngle, Utils.h:235-247: wrap into [-pi, pi).
+// Utils::shiftAngle, Utils.h:235-247: wrap into [-pi, pi).  fmod(x, y)
+// is x itself for |x| < y (exactly, -0.0 included), so the library call is
+// only made outside that range.
 double wrap_angle(double a) {
-    a = std::fmod(a + M_PI, 2.0 * M_PI);
+    const double x = a + M_PI, y = 2.0 * M_PI;
+    a = (x > -y && x < y) ? x : std::fmod(x, y);
     if (a < 0.0) a += 2.0 * M_PI;
     return a - M_PI;
 }
@@ -31,19 +34,28 @@ double dot(const double* a, const double* b, int n) {
 // same values.  kNZ[i] lists the structurally non-zero k of row/column i.
 constexpr int kNZ[6][2] = {{0, 1}, {0, 1}, {2, -1}, {3, 4}, {3, 4}, {5, -1}};
 
-// Transpose(T) * A * T
+// Transpose(T) * A * T (loops fully unrolled, so the sparsity pattern is
+// resolved at compile time)
 void rotate6(const double T[6][6], const double A[6][6], double out[6][6]) {
     double tmp[6][6];
+#pragma GCC unroll 6
     for (int i = 0; i < 6; i++)
+#pragma GCC unroll 6
         for (int j = 0; j < 6; j++) {
             double s = 0.0;
-            for (int q = 0; q < 2 && kNZ[i][q] >= 0; q++) s += T[kNZ[i][q]][i] * A[kNZ[i][q]][j];
+#pragma GCC unroll 2
+            for (int q = 0; q < 2; q++)
+                if (kNZ[i][q] >= 0) s += T[kNZ[i][q]][i] * A[kNZ[i][q]][j];
             tmp[i][j] = s;
         }
+#pragma GCC unroll 6
     for (int i = 0; i < 6; i++)
+#pragma GCC unroll 6
         for (int j = 0; j < 6; j++) {
             double s = 0.0;
-            for (int q = 0; q < 2 && kNZ[j][q] >= 0; q++) s += tmp[i][kNZ[j][q]] * T[kNZ[j][q]][j];
+#pragma GCC unroll 2
+            for (int q = 0; q < 2; q++)
+                if (kNZ[j][q] >= 0) s += tmp[i][kNZ[j][q]] * T[kNZ[j][q]][j];
             out[i][j] = s;
         }
 }

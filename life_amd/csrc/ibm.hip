@@ -285,7 +285,7 @@ __device__ __forceinline__ void band_back_substitute(int first, int n, const dou
 // B: the band (rows of U replace it), y: the right-hand side; both either
 // in LDS or in global memory (LDS_ selects the instantiation, so that every
 // access is compiled for its own address space).  xb: 2R doubles of LDS.
-template <int KB, bool LDS_>
+template <int KB, bool LDS_, bool ONE_WAVE = false>
 __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __restrict__ B, double* __restrict__ y,
                                                     double* __restrict__ eps, double* __restrict__ xb) {
     constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
@@ -363,7 +363,10 @@ __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __
         pl = pl == C - 1 ? 0 : pl + 1;
         __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
+    // one wavefront: its LDS accesses complete in order, so lane 0 sees the
+    // rows of U the other lanes wrote
+    if (ONE_WAVE) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
     if (L == 0) band_back_substitute<KB>(first, n, B, y, eps);
 }
 
@@ -407,6 +410,102 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
     if (c.eps.outside[q.k]) return;
     eps_band_solve_body<KB>(c.eps.first[q.k], c.eps.n[q.k], c.eps.band + c.eps.off[q.k], c.eps.y, c.m.eps, lds_rows,
                             sm, xb);
+}
+
+// ---------------------------------------------------------------- fused recompute
+// The device half of recomputeObjectVals (Objects.cpp:217-227 and the
+// computeEpsilon of flexible bodies, Objects.cpp:235-321) for one flexible
+// body per workgroup, without global round trips between the stages:
+// supports of its markers (to global memory, as k_find_support), spacing
+// from positions staged in LDS (as k_compute_ds), the band of its epsilon
+// system built in LDS (as k_eps_band_entries), the out-of-band check (as
+// k_eps_far_check) and, if the system is banded, the solve by wavefront 0
+// (as k_eps_band_solve).  Same operations in the same order as those
+// kernels.  LDS: n * (3KB + 5) doubles; bodies of up to kFusedMaxMarkers
+// markers.  Used by the serving engine, whose coupling phase runs alone on
+// the device: a 1024-lane workgroup needs a CU's wave slots free at once, so
+// beside a concurrent bulk stream-collide (the single-case step schedule) it
+// waits for that kernel to drain and the separate small-block kernels are
+// faster there.
+constexpr int kFusedMaxMarkers = 256;
+
+template <int KB>
+__device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridScalars& g, int first, int n,
+                                               int* __restrict__ outside, double* __restrict__ sm,
+                                               double* __restrict__ xb, int* __restrict__ flag) {
+    constexpr int W = 3 * KB + 1;
+    double* P = sm;                // 2n positions
+    double* dsl = sm + 2 * n;      // n spacings
+    double* Bl = dsl + n;          // n * W band
+    double* yl = Bl + n * W;       // n right-hand side
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) *flag = 0;
+    for (int k = tid; k < n; k += nt) {
+        find_support_one(m, g, 1, first + k);
+        P[2 * k] = m.pos[2 * (first + k)];
+        P[2 * k + 1] = m.pos[2 * (first + k) + 1];
+    }
+    __syncthreads();
+    // computeDs (IBMNode.cpp:182-204), as compute_ds_body
+    for (int k = tid; k < n; k += nt) {
+        const double x = P[2 * k], y = P[2 * k + 1];
+        double d2min = INFINITY;
+        for (int q = 0; q < n; q++) {
+            if (q == k) continue;
+            const double dx = x - P[2 * q], dy = y - P[2 * q + 1];
+            const double d2 = 0.0 + dx * dx + dy * dy;
+            if (d2 < d2min) d2min = d2;
+        }
+        const double mag = sqrt(d2min) / g.Dx;
+        const double ds = mag < 10.0 ? mag : 10.0;
+        m.ds[first + k] = ds;
+        dsl[k] = ds;
+    }
+    __syncthreads();
+    // band entries (as eps_band_entry, reading the staged positions and ds)
+    const double Dx = g.Dx;
+    for (int t = tid; t < n * W; t += nt) {
+        const int r = t / W, q = t - r * W;
+        const int c = r - KB + q;
+        double a = 0.0;
+        if (q <= 2 * KB && c >= 0 && c < n) {
+            const int mi = first + r;
+            const double xj = P[2 * c] / Dx, yj = P[2 * c + 1] / Dx;
+            const int cnt = m.supp_cnt[mi];
+            for (int s = 0; s < cnt; s++) {
+                const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
+                const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
+                a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
+            }
+            a = a * dsl[c];
+        }
+        Bl[t] = a;
+    }
+    // pairs outside the band that could couple (as eps_far_pair)
+    const double lim = 3.01 * g.Dx;
+    for (int t = tid; t < n * n; t += nt) {
+        const int r = t / n, c = t - r * n;
+        if (c <= r + KB) continue;
+        if (fabs(P[2 * r] - P[2 * c]) < lim && fabs(P[2 * r + 1] - P[2 * c + 1]) < lim) *flag = 1;
+    }
+    __syncthreads();
+    const int far = *flag;
+    if (tid == 0) *outside = far;
+    if (far || tid >= 64) return;
+    eps_band_solve_impl<KB, true, true>(first, n, Bl, yl, m.eps, xb);
+}
+
+constexpr int kFusedThreads = 1024;
+
+template <int KB>
+__global__ __launch_bounds__(kFusedThreads) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
+                                                              const CaseSys* __restrict__ sys) {
+    extern __shared__ double sm[];
+    __shared__ double xb[2 * (KB + 1)];
+    __shared__ int flag;
+    const CaseSys q = sys[blockIdx.x];
+    const CaseDev& c = cases[q.c];
+    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag);
 }
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
@@ -725,6 +824,20 @@ void launch_markers_out_batch(const CaseDev* cases, const int* list, int n, int 
                               hipStream_t s) {
     if (n <= 0 || max_markers <= 0) return;
     k_markers_out_batch<<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list, stage);
+}
+
+// Fused recompute (one workgroup per flexible body); false when a body has
+// more markers than the LDS layout takes (the caller uses the separate
+// kernels then).
+static size_t fused_recompute_lds(int max_n) { return sizeof(double) * (size_t)max_n * (3 * 8 + 5); }
+
+bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
+                                  hipStream_t s) {
+    if (max_n > kFusedMaxMarkers) return false;
+    if (nsys <= 0) return true;
+    k_recompute_body_batch<8><<<(unsigned)nsys, kFusedThreads, fused_recompute_lds(max_n), s>>>(cases, sys);
+    k_eps_outside_gather_batch<<<(unsigned)((nsys + 255) / 256), 256, 0, s>>>(cases, sys, nsys, outside_out);
+    return true;
 }
 
 }  // namespace life

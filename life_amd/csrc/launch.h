@@ -155,7 +155,7 @@ void launch_support_push64_batch(bool cm, const CaseDev* cases, const int* list,
 template <typename T>
 void launch_bc_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_nbc, hipStream_t s);
 template <typename T>
-void launch_convective_batch(const CaseDev* cases, const int* list, int n, bool ordered, hipStream_t s);
+void launch_convective_batch(const CaseDev* cases, const int* list, int n, int max_ny, bool ordered, hipStream_t s);
 // packed marker staging: in = [x, y, vx, vy] per marker, out = [Fx, Fy, eps]
 void launch_markers_in_batch(const CaseDev* cases, const int* list, int n, int max_markers, const double* stage,
                              hipStream_t s);
@@ -168,6 +168,13 @@ void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int m
 // outside_out[q] = 1 for a listed system that is not banded (no solve done)
 void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
                            hipStream_t s);
+// Supports and spacing of the listed systems' markers plus their banded
+// epsilon systems, one workgroup per system (the work of
+// launch_find_support_batch(flexible), launch_compute_ds_batch and
+// launch_eps_band_batch in one launch).  Returns false, launching nothing,
+// if a system has more markers than the fused kernel takes.
+bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
+                                  hipStream_t s);
 template <typename T>
 void launch_interpolate_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);
 void launch_spread_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);

@@ -172,33 +172,40 @@ void launch_bc(const T* src, T* dst, const GridScalars& g, const BCArgs& a, cons
 // FP64 tree reduction is used.
 constexpr int kConvThreads = 1024;
 
+// Outlet velocity of column c = 0..2 from the right wall, row j: stored in
+// ucol (one thread per (c, j), many workgroups).
 template <typename T>
-__device__ inline void convective_speed_block(const T* __restrict__ src, const GridScalars& g,
-                                              const double* __restrict__ fibm, const double* __restrict__ u0,
-                                              int explicit_state, double* __restrict__ ucol,
-                                              double* __restrict__ delU, int ordered, double* red, double& uOutS) {
+__device__ inline void convective_column(const T* __restrict__ src, const GridScalars& g,
+                                         const double* __restrict__ fibm, const double* __restrict__ u0,
+                                         int explicit_state, double* __restrict__ ucol, int t) {
     const int Ny = g.Ny, Nx = g.Nx;
+    if (t >= 3 * Ny) return;
+    const int c = t / Ny, j = t - c * Ny;
     const long long S = g.S;
-    double part = 0.0;
-    for (int j = threadIdx.x; j < Ny; j += blockDim.x) {
-        for (int c = 0; c < 3; c++) {
-            const long long id = (long long)(Nx - 1 - c) * Ny + j;
-            double ux, uy;
-            if (explicit_state) {
-                ux = u0[2 * id];
-                uy = u0[2 * id + 1];
-            } else {
-                double fs[kNV], r;
-                load_site(src, S, id, fs);
-                moments64(fs, g.Fx + fibm[id], g.Fy + fibm[S + id], r, ux, uy);
-            }
-            ucol[(c * 2 + 0) * (long long)Ny + j] = ux;
-            ucol[(c * 2 + 1) * (long long)Ny + j] = uy;
-            if (c == 0) part += ux;
-        }
+    const long long id = (long long)(Nx - 1 - c) * Ny + j;
+    double ux, uy;
+    if (explicit_state) {
+        ux = u0[2 * id];
+        uy = u0[2 * id + 1];
+    } else {
+        double fs[kNV], r;
+        load_site(src, S, id, fs);
+        moments64(fs, g.Fx + fibm[id], g.Fy + fibm[S + id], r, ux, uy);
     }
+    ucol[(c * 2 + 0) * (long long)Ny + j] = ux;
+    ucol[(c * 2 + 1) * (long long)Ny + j] = uy;
+}
+
+// uOut and delU from ucol, one workgroup of kConvThreads.  ordered: the
+// reference's serial left-to-right mean; otherwise each thread sums its
+// rows j = tid, tid + kConvThreads, ... in increasing j and a fixed tree
+// combines the threads.
+__device__ inline void convective_finish(const GridScalars& g, const double* __restrict__ ucol,
+                                         double* __restrict__ delU, int ordered, double* red, double& uOutS) {
+    const int Ny = g.Ny;
+    double part = 0.0;
+    for (int j = threadIdx.x; j < Ny; j += blockDim.x) part += ucol[j];
     red[threadIdx.x] = part;
-    __threadfence_block();
     __syncthreads();
     if (ordered) {
         if (threadIdx.x == 0) {
@@ -226,33 +233,43 @@ __device__ inline void convective_speed_block(const T* __restrict__ src, const G
 }
 
 template <typename T>
-__global__ __launch_bounds__(kConvThreads) void k_convective_speed(const T* __restrict__ src, GridScalars g,
-                                                                   const double* __restrict__ fibm,
-                                                                   const double* __restrict__ u0, int explicit_state,
-                                                                   double* __restrict__ ucol,
-                                                                   double* __restrict__ delU, int ordered) {
-    __shared__ double red[kConvThreads];
-    __shared__ double uOutS;
-    convective_speed_block(src, g, fibm, u0, explicit_state, ucol, delU, ordered, red, uOutS);
+__global__ __launch_bounds__(256) void k_convective_cols(const T* __restrict__ src, GridScalars g,
+                                                         const double* __restrict__ fibm,
+                                                         const double* __restrict__ u0, int explicit_state,
+                                                         double* __restrict__ ucol) {
+    convective_column(src, g, fibm, u0, explicit_state, ucol, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Serving: one workgroup per listed case (cases with a convective outlet).
+__global__ __launch_bounds__(kConvThreads) void k_convective_finish(GridScalars g, const double* __restrict__ ucol,
+                                                                    double* __restrict__ delU, int ordered) {
+    __shared__ double red[kConvThreads];
+    __shared__ double uOutS;
+    convective_finish(g, ucol, delU, ordered, red, uOutS);
+}
+
+// Serving: grid.y = listed case (cases with a convective outlet).
 template <typename T>
-__global__ __launch_bounds__(kConvThreads) void k_convective_batch(const CaseDev* __restrict__ cases,
-                                                                   const int* __restrict__ list, int ordered) {
+__global__ __launch_bounds__(256) void k_convective_cols_batch(const CaseDev* __restrict__ cases,
+                                                               const int* __restrict__ list) {
+    const CaseDev& c = cases[list[blockIdx.y]];
+    convective_column(static_cast<const T*>(c.src), c.g, c.fibm, c.u0, c.explicit_state, c.ucol,
+                      blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(kConvThreads) void k_convective_finish_batch(const CaseDev* __restrict__ cases,
+                                                                          const int* __restrict__ list, int ordered) {
     __shared__ double red[kConvThreads];
     __shared__ double uOutS;
     const CaseDev& c = cases[list[blockIdx.x]];
-    convective_speed_block(static_cast<const T*>(c.src), c.g, c.fibm, c.u0, c.explicit_state, c.ucol,
-                           const_cast<double*>(c.bc.delU),
-                           ordered, red, uOutS);
+    convective_finish(c.g, c.ucol, const_cast<double*>(c.bc.delU), ordered, red, uOutS);
 }
 
 template <typename T>
 void launch_convective_speed(const T* src, const GridScalars& g, const double* fibm, const double* u0,
                              bool explicit_state, double* ucol, double* delU, bool ordered, hipStream_t s) {
-    k_convective_speed<T><<<1, kConvThreads, 0, s>>>(src, g, fibm, u0, explicit_state ? 1 : 0, ucol, delU,
-                                                     ordered ? 1 : 0);
+    k_convective_cols<T><<<(unsigned)((3 * g.Ny + 255) / 256), 256, 0, s>>>(src, g, fibm, u0, explicit_state ? 1 : 0,
+                                                                           ucol);
+    k_convective_finish<<<1, kConvThreads, 0, s>>>(g, ucol, delU, ordered ? 1 : 0);
 }
 
 template <typename T>
@@ -264,9 +281,10 @@ void launch_bc_batch(bool cm, const CaseDev* cases, const int* list, int n, int 
 }
 
 template <typename T>
-void launch_convective_batch(const CaseDev* cases, const int* list, int n, bool ordered, hipStream_t s) {
-    if (n <= 0) return;
-    k_convective_batch<T><<<(unsigned)n, kConvThreads, 0, s>>>(cases, list, ordered ? 1 : 0);
+void launch_convective_batch(const CaseDev* cases, const int* list, int n, int max_ny, bool ordered, hipStream_t s) {
+    if (n <= 0 || max_ny <= 0) return;
+    k_convective_cols_batch<T><<<dim3((unsigned)((3 * max_ny + 255) / 256), (unsigned)n), 256, 0, s>>>(cases, list);
+    k_convective_finish_batch<<<(unsigned)n, kConvThreads, 0, s>>>(cases, list, ordered ? 1 : 0);
 }
 
 // f = f_eq(rho, u) at every site (Grid.cpp:1048-1061).
@@ -423,7 +441,7 @@ void launch_init_moments(double* rho0, double* u0, const GridScalars& g, int mod
                                         bool, double*, double*, double*, double*, hipStream_t);                      \
     template void launch_upload<T>(T*, const GridScalars&, const double*, hipStream_t);                             \
     template void launch_bc_batch<T>(bool, const CaseDev*, const int*, int, int, hipStream_t);                       \
-    template void launch_convective_batch<T>(const CaseDev*, const int*, int, bool, hipStream_t);
+    template void launch_convective_batch<T>(const CaseDev*, const int*, int, int, bool, hipStream_t);
 LIFE_INST(double)
 LIFE_INST(float)
 

@@ -82,27 +82,43 @@ void BandMatrix::resize(int n_, int kl_, int ku_) {
 
 void BandMatrix::zero() { std::memset(data.data(), 0, sizeof(double) * data.size()); }
 
+namespace {
+// row_i[j] -= l * row_k[j] for j in [lo, hi] (rows k != i: no aliasing)
+inline void axpy_row(double* __restrict ri, const double* __restrict rk, double l, int lo, int hi) {
+    for (int j = lo; j <= hi; j++) ri[j] = ri[j] - l * rk[j];
+}
+}  // namespace
+
 int lu_solve_band(const BandMatrix& A, int bc, const double* b, double* x, BandMatrix& W, std::vector<double>& y) {
     const int n = A.n, kl = A.kl, ku = A.ku;
     const int m = n - bc;
     if (W.n != m || W.kl != kl || W.ku != ku) W.resize(m, kl, ku);
     W.zero();
+    // row(r)[c] is entry (r, c) for c in [r - kl, r + kl + ku]
+    const int w = W.w;
+    double* const base = W.data.data();
+    auto row = [&](int r) { return base + (ptrdiff_t)r * w + kl - r; };
+    const double* const abase = A.data.data();
+    auto arow = [&](int r) { return abase + (ptrdiff_t)r * A.w + kl - r; };
     // copy the trailing block; entries beyond the original upper band start at zero
     for (int r = 0; r < m; r++) {
         const int lo = r - kl < 0 ? 0 : r - kl;
         const int hi = r + ku > m - 1 ? m - 1 : r + ku;
-        for (int c = lo; c <= hi; c++) W.at(r, c) = A.at(r + bc, c + bc);
+        double* wr = row(r);
+        const double* ar = arow(r + bc) + bc;
+        for (int c = lo; c <= hi; c++) wr[c] = ar[c];
     }
     y.resize(m);
-    for (int r = 0; r < m; r++) y[r] = b[r + bc];
+    double* const yv = y.data();
+    for (int r = 0; r < m; r++) yv[r] = b[r + bc];
     const int kext = kl + ku;
     int info = 0;
     for (int k = 0; k < m; k++) {
         const int ilast = k + kl > m - 1 ? m - 1 : k + kl;
         int p = k;
-        double best = std::fabs(W.at(k, k));
+        double best = std::fabs(row(k)[k]);
         for (int i = k + 1; i <= ilast; i++) {
-            const double a = std::fabs(W.at(i, k));
+            const double a = std::fabs(row(i)[k]);
             if (a > best) {
                 best = a;
                 p = i;
@@ -113,29 +129,33 @@ int lu_solve_band(const BandMatrix& A, int bc, const double* b, double* x, BandM
             continue;
         }
         const int jlast = k + kext > m - 1 ? m - 1 : k + kext;
+        double* rk = row(k);
         if (p != k) {
+            double* rp = row(p);
             for (int j = k; j <= jlast; j++) {
-                const double t = W.at(k, j);
-                W.at(k, j) = W.at(p, j);
-                W.at(p, j) = t;
+                const double t = rk[j];
+                rk[j] = rp[j];
+                rp[j] = t;
             }
-            const double t = y[k];
-            y[k] = y[p];
-            y[p] = t;
+            const double t = yv[k];
+            yv[k] = yv[p];
+            yv[p] = t;
         }
-        const double piv = W.at(k, k);
+        const double piv = rk[k];
         for (int i = k + 1; i <= ilast; i++) {
-            const double l = W.at(i, k) / piv;
-            W.at(i, k) = l;
-            for (int j = k + 1; j <= jlast; j++) W.at(i, j) = W.at(i, j) - l * W.at(k, j);
-            y[i] = y[i] - l * y[k];
+            double* ri = row(i);
+            const double l = ri[k] / piv;
+            ri[k] = l;
+            axpy_row(ri, rk, l, k + 1, jlast);
+            yv[i] = yv[i] - l * yv[k];
         }
     }
     for (int i = m - 1; i >= 0; i--) {
-        double s = y[i];
+        const double* ri = row(i);
+        double s = yv[i];
         const int jlast = i + kext > m - 1 ? m - 1 : i + kext;
-        for (int j = i + 1; j <= jlast; j++) s = s - W.at(i, j) * x[j + bc];
-        x[i + bc] = s / W.at(i, i);
+        for (int j = i + 1; j <= jlast; j++) s = s - ri[j] * x[j + bc];
+        x[i + bc] = s / ri[i];
     }
     for (int i = 0; i < bc; i++) x[i] = 0.0;
     return info;

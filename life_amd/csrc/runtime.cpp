@@ -217,6 +217,10 @@ void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const s
     g->eps_n = dalloc<int>(first.size());
     g->eps_off = dalloc<long long>(first.size());
     g->eps_outside = dalloc<int>(first.size());
+    if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
+    g->eps_outside_h = nullptr;
+    LIFE_HIP(hipHostMalloc((void**)&g->eps_outside_h, sizeof(int) * (first.empty() ? 1 : first.size()),
+                           hipHostMallocDefault));
     g->eps_band = dalloc<double>((size_t)tot);
     g->eps_y = dalloc<double>((size_t)(g->md.n > 0 ? g->md.n : 1));
     if (!first.empty()) {
@@ -229,18 +233,20 @@ void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const s
     LIFE_HIP(hipStreamSynchronize(g->stream));
 }
 
-std::vector<int> ibm_epsilon_band(life_grid* g) {
-    std::vector<int> dense;
-    if (g->eps_nsys <= 0) return dense;
+void ibm_epsilon_band_launch(life_grid* g) {
+    if (g->eps_nsys <= 0) return;
     launch_fill_i32(g->eps_outside, 0, g->eps_nsys, g->stream);
     launch_eps_band(g->md, g->g, g->eps_first, g->eps_n, g->eps_n_h.data(), g->eps_nsys, g->eps_KB, g->eps_band, g->eps_off,
                     g->eps_outside, g->eps_y, g->stream);
     LIFE_HIP(hipGetLastError());
-    std::vector<int> out(g->eps_nsys);
-    LIFE_HIP(hipMemcpyAsync(out.data(), g->eps_outside, sizeof(int) * out.size(), hipMemcpyDeviceToHost, g->stream));
-    LIFE_HIP(hipStreamSynchronize(g->stream));
+    LIFE_HIP(hipMemcpyAsync(g->eps_outside_h, g->eps_outside, sizeof(int) * g->eps_nsys, hipMemcpyDeviceToHost,
+                            g->stream));
+}
+
+std::vector<int> ibm_epsilon_band_dense(const life_grid* g) {
+    std::vector<int> dense;
     for (int k = 0; k < g->eps_nsys; k++)
-        if (out[k]) dense.push_back(k);
+        if (g->eps_outside_h[k]) dense.push_back(k);
     return dense;
 }
 
@@ -431,6 +437,7 @@ int life_grid_destroy(life_grid* g) {
                         (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
         if (g->stream2) (void)hipStreamSynchronize(g->stream2);
+        if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
         for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk})
             if (e) (void)hipEventDestroy(e);
         if (g->stream2) (void)hipStreamDestroy(g->stream2);

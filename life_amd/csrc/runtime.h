@@ -88,6 +88,7 @@ struct life_grid {
     int* eps_n = nullptr;
     long long* eps_off = nullptr;
     int* eps_outside = nullptr;
+    int* eps_outside_h = nullptr;  // page-locked copy of eps_outside
     double* eps_band = nullptr;
     double* eps_y = nullptr;
     std::vector<int> eps_first_h, eps_n_h;
@@ -119,8 +120,12 @@ void ibm_find_support(life_grid* g, int flexible_only);
 void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host);
 // Declare the banded epsilon systems [first[k], first[k]+n[k]) once.
 void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const std::vector<int>& n, int KB);
-// Assemble and solve them on the device (epsilon lands in md.eps); returns
-// the indices of systems that were not banded and still need a solve.
-std::vector<int> ibm_epsilon_band(life_grid* g);
+// Assemble and solve them on the device (epsilon lands in md.eps) and queue
+// the copy of their out-of-band flags to the host; does not wait.
+void ibm_epsilon_band_launch(life_grid* g);
+// After the grid stream has been synchronised past ibm_epsilon_band_launch:
+// the indices of systems that were not banded (not solved; their epsilon
+// still needs a solve).
+std::vector<int> ibm_epsilon_band_dense(const life_grid* g);
 void check_flags(life_grid* g);
 }  // namespace life

@@ -149,6 +149,7 @@ struct LatticePlan {
     std::vector<CaseBlock> blocks[2];  // BGK, central moments
     std::vector<int> conv, ibm[2], bc[2];
     int max_sites[2] = {0, 0}, max_nbc[2] = {0, 0};
+    int max_ny = 0;  // largest Ny of the cases with a convective outlet
 };
 
 }  // namespace
@@ -240,7 +241,10 @@ void rebuild(life_server* s) {
         const int bs = s->cfg.precision == LIFE_FP64 ? kServeBlock64 : kServeBlock32;
         const long long nb = (g->g.N + bs - 1) / bs;
         for (long long b = 0; b < nb; b++) P.blocks[cm].push_back(CaseBlock{q, (int)b});
-        if (g->convective) P.conv.push_back(q);
+        if (g->convective) {
+            P.conv.push_back(q);
+            P.max_ny = std::max(P.max_ny, g->g.Ny);
+        }
         if (g->nbc > 0) {
             P.bc[cm].push_back(q);
             P.max_nbc[cm] = std::max(P.max_nbc[cm], g->nbc);
@@ -318,8 +322,10 @@ void launch_lattice(life_server* s) {
     hipStream_t st = s->stream;
     const int* L = s->lists.d;
     if (!P.conv.empty()) {
-        if (f64) launch_convective_batch<double>(s->cases.d, L + s->off_conv, (int)P.conv.size(), true, st);
-        else launch_convective_batch<float>(s->cases.d, L + s->off_conv, (int)P.conv.size(), false, st);
+        if (f64)
+            launch_convective_batch<double>(s->cases.d, L + s->off_conv, (int)P.conv.size(), P.max_ny, true, st);
+        else
+            launch_convective_batch<float>(s->cases.d, L + s->off_conv, (int)P.conv.size(), P.max_ny, false, st);
     }
     for (int k = 0; k < 2; k++) {
         const int nb = (int)P.blocks[k].size();
@@ -441,13 +447,13 @@ void coupling_phase(life_server* s) {
             s->times.lap(st, 2);
             markers_in(s, active);
             const int na = (int)active.size();
-            launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
-            launch_compute_ds_batch(s->cases.d, s->dyn.d, na, s->max_bodies, 1, st);
             int max_n = 0;
+            bool any_uni = false;
             for (int q : active) {
                 life_case* c = s->slots[q].c;
                 life_grid* g = c->g;
                 if (c->p.uni_epsilon) {
+                    any_uni = true;
                     dense.push_back({q, 0});
                     dense_n.push_back(c->nm);
                     continue;
@@ -462,9 +468,19 @@ void coupling_phase(life_server* s) {
                 s->outside.reserve(sysl.size());
                 std::copy(sysl.begin(), sysl.end(), s->sys.h);
                 s->sys.upload(sysl.size(), st);
-                launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, st);
-                s->outside.download(sysl.size(), st);
             }
+            // one workgroup per flexible body for supports, spacing and the
+            // banded epsilon system when every case can take that path
+            const bool fused = !any_uni && !sysl.empty() &&
+                               launch_recompute_fused_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n,
+                                                            s->outside.d, st);
+            if (!fused) {
+                launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
+                launch_compute_ds_batch(s->cases.d, s->dyn.d, na, s->max_bodies, 1, st);
+                if (!sysl.empty())
+                    launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, st);
+            }
+            if (!sysl.empty()) s->outside.download(sysl.size(), st);
         }
         // ibmKernelInterp: every case with bodies on the first pass, then
         // the cases still subiterating
@@ -546,7 +562,7 @@ void retire(life_server* s, Slot& sl) {
                 r->u.resize(2 * N);
                 if (life_grid_download(c->g, nullptr, r->rho.data(), r->u.data(), nullptr))
                     throw std::runtime_error(life_last_error());
-                r->mpos = c->pos;
+                r->mpos.assign(c->pos.begin(), c->pos.end());
             }
             r->info.status = LIFE_REQ_DONE;
         } catch (const HipError& e) {

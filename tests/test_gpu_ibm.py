@@ -110,6 +110,22 @@ def test_canopy_small(oracle, gpu_lib):
     compare_fields(oc, gc)
 
 
+def test_long_filament(oracle, gpu_lib):
+    """A 301-marker filament: more markers than the fused per-body kernel
+    and the LDS-staged solve take, so supports, spacing and the banded
+    epsilon solve run as separate kernels with the band in global memory."""
+    p, g = cfg.canopy(Nx=200, Ny=340, n_filaments=1, L=300.0, n_el=30, min_spacing=30.0, x_range=(60, 120),
+                      ramp_steps=0.0, u_in=0.05, kb_range=(5e-2, 5e-2))
+    p = p.replace(ux0_p=0.05)
+    oc, gc = build(oracle, p, g)
+    assert oc.n_markers == 301
+    oc.step(6)
+    gc.step(6)
+    assert rel_err(gc.fem(0)[0], oc.fem(0)[0]) <= TOL
+    compare_markers(oc, gc)
+    compare_fields(oc, gc)
+
+
 def test_inverted_flag_initial_deflection(oracle, gpu_lib):
     """INITIAL_DEFLECT (the reference's InvertedFlag example): static FEM at
     creation, then the coupled steps, against the oracle."""
