@@ -91,6 +91,7 @@ struct life_case {
     // recorded after the last upload of pos/vel in a step; waited on before
     // the host next writes them
     hipEvent_t ev_upload = nullptr;
+    hipEvent_t ev_forces = nullptr;  // marker forces/epsilon downloaded
     // banded epsilon systems were launched; their out-of-band flags are
     // read at the next synchronisation (the force download)
     bool eps_check_pending = false;

@@ -186,19 +186,29 @@ void compute_epsilon(life_case* c, bool initial) {
 
 // Dense solves of the systems that the banded path left (after a
 // synchronisation of the grid stream).
-void finish_epsilon(life_case* c) {
-    if (!c->eps_check_pending) return;
+bool finish_epsilon(life_case* c) {
+    if (!c->eps_check_pending) return false;
     c->eps_check_pending = false;
-    for (int k : ibm_epsilon_band_dense(c->g)) case_solve_eps_dense(c, c->g->eps_first_h[k], c->g->eps_n_h[k]);
+    const std::vector<int> dense = ibm_epsilon_band_dense(c->g);
+    for (int k : dense) case_solve_eps_dense(c, c->g->eps_first_h[k], c->g->eps_n_h[k]);
+    return !dense.empty();
 }
 
-// Marker forces (and epsilon, which the FEM load vector uses) to the host.
-void download_forces(life_case* c) {
-    LIFE_HIP(hipMemcpyAsync(c->force.data(), c->g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
-                            c->g->stream));
-    LIFE_HIP(hipMemcpyAsync(c->eps.data(), c->g->md.eps, sizeof(double) * c->nm, hipMemcpyDeviceToHost, c->g->stream));
-    LIFE_HIP(hipStreamSynchronize(c->g->stream));
-    finish_epsilon(c);
+// Download the marker forces and epsilon; returns once they are on the
+// host.  With spread set, the spread is queued behind the copies and runs
+// on the device while the host goes on (the FEM does not change what it
+// reads: supports, forces, epsilon); the return value says whether that
+// spread used the final epsilon (false after a dense fallback solve, which
+// rewrites epsilon).
+bool download_forces(life_case* c, bool spread) {
+    life_grid* g = c->g;
+    LIFE_HIP(hipMemcpyAsync(c->force.data(), g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
+                            g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->eps.data(), g->md.eps, sizeof(double) * c->nm, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipEventRecord(c->ev_forces, g->stream));
+    if (spread) ibm_spread(g);
+    LIFE_HIP(hipEventSynchronize(c->ev_forces));
+    return !finish_epsilon(c) && spread;
 }
 
 // Run fn(body) for every flexible body.  Bodies are independent (disjoint
@@ -290,6 +300,7 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
     c->subIt = 0;
     c->forces_on_host = false;
     PhaseTimes& T = *c->timing;
+    bool spread_final = false, more = false;
     do {
         if (c->hasFlex) recompute_object_vals(c);
         T.lap(c->g, 1);
@@ -298,13 +309,18 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
         // pass of the coupling kernels is queued, beside the host FEM
         if (mode == 2) launch_pending(c, eb);
         if (!c->hasFlex) break;
-        download_forces(c);
+        // the spread with these forces starts beside the host FEM: if the
+        // FEM ends the subiterations they are the final forces; otherwise
+        // the spread is done again below, and a spread overwrites every
+        // site of the previous one, so the result is the same
+        spread_final = download_forces(c, true);
         T.lap(c->g, 2);
         T.subits++;
         case_fem(c);
         T.lap(c->g, 3);
-    } while (case_subiterate(c));
-    ibm_spread(c->g);
+        more = case_subiterate(c);
+    } while (more);
+    if (!spread_final) ibm_spread(c->g);
     T.lap(c->g, 4);
     if (c->hasFlex) {
         // the last FEM solve moved the markers (FEMBody.cpp:63-64); the
@@ -384,6 +400,7 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
             for (Body* b : c->flex_bodies) b->fem->set_initial_deflection(p->initial_deflect, c->view(*b));
         if (life_grid_create(p, precision, device, &c->g)) throw std::runtime_error(life_last_error());
         LIFE_HIP(hipEventCreateWithFlags(&c->ev_upload, hipEventDisableTiming));
+        LIFE_HIP(hipEventCreateWithFlags(&c->ev_forces, hipEventDisableTiming));
         if (c->hasIBM) {
             std::vector<int> flex(c->bodies.size());
             for (size_t b = 0; b < c->bodies.size(); b++) flex[b] = c->bodies[b].flex;
@@ -424,6 +441,7 @@ int life_case_destroy(life_case* c) {
         (void)hipSetDevice(c->g->device);
         for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
         if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+        if (c->ev_forces) (void)hipEventDestroy(c->ev_forces);
         rc = life_grid_destroy(c->g);
     }
     delete c;
@@ -434,6 +452,7 @@ int life_case_step(life_case* c, int nsteps) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(c->g->device));
         for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr, s + 1 < nsteps, nullptr, nullptr);
+        lbm_join(c->g);
         LIFE_HIP(hipStreamSynchronize(c->g->stream));
         if (c->hasIBM) check_flags(c->g);
     });
@@ -461,6 +480,7 @@ int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_
             case_step(c, early ? nullptr : c->kev[2 * s], early ? nullptr : c->kev[2 * s + 1], s + 1 < nsteps, nb,
                       ne);
         }
+        lbm_join(g);
         LIFE_HIP(hipEventRecord(g->ev_b, g->stream));
         LIFE_HIP(hipEventSynchronize(g->ev_b));
         float ms = 0.0f;

@@ -223,11 +223,15 @@ void FemBody::update_fem() {
 }
 
 void FemBody::update_ibm(MarkerView mv) {
-    for (int i = 0; i < mv.n; i++) {
-        const Elem& e = el_[pmap_[i].el];
-        const double zeta = pmap_[i].zeta;
+    // An element's displacement and velocity in its local frame do not
+    // depend on the marker: computed once per element, then interpolated
+    // to each of its markers (the same operations as per marker).
+    const int nel = (int)el_.size();
+    elu_.resize((size_t)nel * 12);
+    for (int q = 0; q < nel; q++) {
+        const Elem& e = el_[q];
         const int d0 = 3 * e.n0;
-        double dU[6], dUd[6], lU[6], lUd[6];
+        double dU[6], dUd[6];
         for (int k = 0; k < 6; k++) {
             dU[k] = U[d0 + k];
             dUd[k] = Ud[d0 + k];
@@ -239,17 +243,23 @@ void FemBody::update_ibm(MarkerView mv) {
             dU[n * 3 + 2] += nn[n]->angle0 - e.angle;
             dU[n * 3 + 2] = wrap_angle(dU[n * 3 + 2]);
         }
-        mul6(e.T, dU, lU);
-        mul6(e.T, dUd, lUd);
+        mul6(e.T, dU, &elu_[(size_t)q * 12]);
+        mul6(e.T, dUd, &elu_[(size_t)q * 12 + 6]);
+    }
+    for (int i = 0; i < mv.n; i++) {
+        const int q = pmap_[i].el;
+        const Elem& e = el_[q];
+        const double* lU = &elu_[(size_t)q * 12];
         double s[2], sd[2];
-        shape(e, lU, zeta, s);
-        shape(e, lUd, zeta, sd);
+        shape(e, lU, pmap_[i].zeta, s);
+        shape(e, lU + 6, pmap_[i].zeta, sd);
+        const double* p0 = node_[e.n0].pos;
         double* p = mv.pos + 2 * (mv.first + i);
         double* v = mv.vel + 2 * (mv.first + i);
         for (int r = 0; r < 2; r++) {
             const double gs = 0.0 + e.T[0][r] * s[0] + e.T[1][r] * s[1];
             const double gsd = 0.0 + e.T[0][r] * sd[0] + e.T[1][r] * sd[1];
-            p[r] = nn[0]->pos[r] + gs;
+            p[r] = p0[r] + gs;
             v[r] = gsd;
         }
     }

@@ -98,6 +98,7 @@ class FemBody {
     std::vector<Node> node_;
     std::vector<Elem> el_;
     std::vector<PosMap> pmap_;
+    std::vector<double> elu_;  // per element: local displacement, velocity (update_ibm)
     BandMatrix M_, K_, W_;
     std::vector<double> vec_, y_;
 };

@@ -228,6 +228,14 @@ __global__ void k_eps_far_check_batch(const CaseDev* __restrict__ cases, const C
 // equal the dense elimination's.  A single wavefront is latency-bound
 // here: every step is one dependent chain (search, division, update).
 
+// Orders LDS writes of some lanes before later reads by other lanes of the
+// same wavefront (LDS operations of a wavefront complete in order; the
+// fence stops the compiler from reordering them across lanes).
+__device__ __forceinline__ void lane_exchange_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Lane `lane`'s value as a wavefront-uniform (SGPR) value.
 __device__ inline double uniform64(double v, int lane) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -237,20 +245,21 @@ __device__ inline double uniform64(double v, int lane) {
 }
 
 // Back substitution on one lane in increasing column order, two rows per
-// iteration (no register shuffling between them).  Terms past n are 0 * 0
+// iteration (no register shuffling between them); rows of U stored with
+// half-bandwidth KBS, window half-bandwidth KB.  Terms past n are 0 * 0
 // (structural zeros of U times the zero-initialised x window), which
 // leave s unchanged, so rows need no bounds tests.
-template <int KB>
+template <int KB, int KBS>
 __device__ __forceinline__ void band_back_substitute(int first, int n, const double* __restrict__ B,
                                                      const double* __restrict__ y, double* __restrict__ eps) {
-    constexpr int W = 3 * KB + 1;
+    constexpr int W = 3 * KBS + 1;
     double xa[2 * KB + 1];  // xa[q] = x[i+1+q] for the row pair (i, i-1)
 #pragma unroll
     for (int q = 0; q <= 2 * KB; q++) xa[q] = 0.0;
     int i = n - 1;
     for (; i >= 1; i -= 2) {
-        const double* U1 = B + i * W + KB;
-        const double* U0 = B + (i - 1) * W + KB;
+        const double* U1 = B + i * W + KBS;
+        const double* U0 = B + (i - 1) * W + KBS;
         double u1[2 * KB + 1], u0[2 * KB + 1];
 #pragma unroll
         for (int q = 0; q <= 2 * KB; q++) {
@@ -274,7 +283,7 @@ __device__ __forceinline__ void band_back_substitute(int first, int n, const dou
         xa[0] = x0;
     }
     if (i == 0) {
-        const double* U = B + KB;
+        const double* U = B + KBS;
         double s = y[0];
 #pragma unroll
         for (int q = 0; q < 2 * KB; q++) s = s - U[1 + q] * xa[q];
@@ -285,29 +294,36 @@ __device__ __forceinline__ void band_back_substitute(int first, int n, const dou
 // B: the band (rows of U replace it), y: the right-hand side; both either
 // in LDS or in global memory (LDS_ selects the instantiation, so that every
 // access is compiled for its own address space).  xb: 2R doubles of LDS.
-template <int KB, bool LDS_, bool ONE_WAVE = false>
+// The band is stored with half-bandwidth KBS (W = 3KBS + 1 per row); the
+// elimination window has half-bandwidth KB <= KBS, which must bound every
+// non-zero of the matrix (|column - row| <= KB).
+template <int KB, int KBS, bool LDS_, bool ONE_WAVE = false>
 __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __restrict__ B, double* __restrict__ y,
                                                     double* __restrict__ eps, double* __restrict__ xb) {
-    constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KB + 1;
+    static_assert(KB <= KBS, "window wider than the stored band");
+    constexpr int R = KB + 1, C = 2 * KB + 1, W = 3 * KBS + 1;
     const int L = threadIdx.x;
-    auto A0 = [&](int r, int col) -> double {
-        if (L >= C || r >= n || col < 0 || col >= n) return 0.0;
-        const int q = col - r + KB;
-        return (q >= 0 && q < W) ? B[r * W + q] : 0.0;
-    };
-    double w[R], yw[R];
+    // lanes 0..C-1 hold the window columns; lane C holds the right-hand
+    // side, which the row operations treat as one more column
+    const bool rhs_lane = L == C;
+    double w[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        w[r] = A0(r, L);
-        yw[r] = r < n ? 1.0 : 0.0;
+        const int q = L - r + KBS;
+        const bool in = L < C && r < n && L < n && q >= 0 && q < W;
+        const double a = B[in ? r * W + q : 0];
+        w[r] = in ? a : (rhs_lane && r < n ? 1.0 : 0.0);
     }
     int pl = 0;   // lane holding column k
     int off = L;  // this lane holds column k + off
     for (int k = 0; k < n; k++) {
         // row k+R enters this lane's column after the step (band offset
-        // q = column - row + KB, always inside the stored band here)
+        // q = column - row + KBS, always inside the stored band here); the
+        // load is unconditional (clamped address) so that its latency
+        // overlaps the step
         const int cn = L == pl ? C : off;
-        const double incoming = (L < C && k + R < n && k + cn < n) ? B[(k + R) * W + (cn - 1)] : 0.0;
+        const bool in = L < C && k + R < n && k + cn < n;
+        const double a_in = B[in ? (k + R) * W + (cn - R + KBS) : 0];
         double best = fabs(w[0]);
         int pr = 0;
 #pragma unroll
@@ -318,72 +334,123 @@ __device__ __forceinline__ void eps_band_solve_impl(int first, int n, double* __
             pr = gt ? r : pr;
         }
         const int p = __builtin_amdgcn_readlane(pr, pl);
-        if (p != 0) {
-            const double tw = w[0], ty = yw[0];
+        if (__builtin_expect(p != 0, 0)) {
+            const double tw = w[0];
 #pragma unroll
             for (int r = 1; r < R; r++)
                 if (r == p) {
                     w[0] = w[r];
                     w[r] = tw;
-                    yw[0] = yw[r];
-                    yw[r] = ty;
                 }
         }
-        if (L == pl) {
-#pragma unroll
-            for (int r = 0; r < R; r++) xb[r] = w[r];
-        }
-        __builtin_amdgcn_wave_barrier();
-        const double piv0 = xb[0];
-        const double piv = piv0 != 0.0 ? piv0 : 1.0;  // zero column: all multipliers 0
-        const double num = (L >= 1 && L < R) ? xb[L] : 0.0;
-        if (L >= 1 && L < R) xb[R + L] = num / piv;
-        __builtin_amdgcn_wave_barrier();
         double l[R];
+        if constexpr (KB <= 4) {
+            // narrow window: column k and the multipliers as wavefront-
+            // uniform values (v_readlane), no LDS round trip
+            const double piv0 = uniform64(w[0], pl);
+            const double piv = piv0 != 0.0 ? piv0 : 1.0;  // zero column: all multipliers 0
+            double num = 0.0;
 #pragma unroll
-        for (int r = 1; r < R; r++) l[r] = xb[R + r];
+            for (int r = 1; r < R; r++) {
+                const double a = uniform64(w[r], pl);
+                num = L == r ? a : num;
+            }
+            const double lm = num / piv;
+#pragma unroll
+            for (int r = 1; r < R; r++) l[r] = uniform64(lm, r);
+        } else {
+            if (L == pl) {
+#pragma unroll
+                for (int r = 0; r < R; r++) xb[r] = w[r];
+            }
+            // other lanes read what lane pl wrote: a wavefront-scope fence
+            // keeps the compiler from moving their reads above the write
+            lane_exchange_fence();
+            const double piv0 = xb[0];
+            const double piv = piv0 != 0.0 ? piv0 : 1.0;  // zero column: all multipliers 0
+            const double num = (L >= 1 && L < R) ? xb[L] : 0.0;
+            if (L >= 1 && L < R) xb[R + L] = num / piv;
+            lane_exchange_fence();
+#pragma unroll
+            for (int r = 1; r < R; r++) l[r] = xb[R + r];
+        }
 #pragma unroll
         for (int r = 1; r < R; r++) {
             const double upd = w[r] - l[r] * w[0];
             w[r] = L == pl ? 0.0 : upd;
-            yw[r] = yw[r] - l[r] * yw[0];
         }
         // row k of U (in place of row k of the band) and the final y_k
-        if (L < C && k + off < n) B[k * W + (off + KB)] = w[0];
-        if (L == 0) y[k] = yw[0];
+        if (L < C && k + off < n) B[k * W + (off + KBS)] = w[0];
+        if (rhs_lane) y[k] = w[0];
         // drop row k, take in row k+R; column k's lane takes column k+C
 #pragma unroll
-        for (int r = 0; r < R - 1; r++) {
-            w[r] = w[r + 1];
-            yw[r] = yw[r + 1];
-        }
-        w[R - 1] = incoming;
-        yw[R - 1] = k + R < n ? 1.0 : 0.0;
+        for (int r = 0; r < R - 1; r++) w[r] = w[r + 1];
+        w[R - 1] = in ? a_in : (rhs_lane && k + R < n ? 1.0 : 0.0);
         off = L == pl ? C - 1 : off - 1;
         pl = pl == C - 1 ? 0 : pl + 1;
-        __builtin_amdgcn_wave_barrier();
+        if constexpr (KB > 4) lane_exchange_fence();  // xb is rewritten next step
+        else __builtin_amdgcn_wave_barrier();
     }
-    // one wavefront: its LDS accesses complete in order, so lane 0 sees the
-    // rows of U the other lanes wrote
-    if (ONE_WAVE) __builtin_amdgcn_wave_barrier();
+    // lane 0 reads the rows of U and y the other lanes wrote
+    if (ONE_WAVE) lane_exchange_fence();
     else __syncthreads();
-    if (L == 0) band_back_substitute<KB>(first, n, B, y, eps);
+    if (L == 0) band_back_substitute<KB, KBS>(first, n, B, y, eps);
 }
 
+// Largest |column - row| over the non-zero entries of rows [r0, r1) of a
+// band stored with half-bandwidth KBS (entries q = 0..2KBS of each row).
+template <int KBS>
+__device__ __forceinline__ int band_width_rows(const double* __restrict__ B, long long t0, long long t1, int stride) {
+    constexpr int W = 3 * KBS + 1;
+    int bw = 0;
+    for (long long t = t0; t < t1; t += stride) {
+        const int q = (int)(t % W);
+        if (q <= 2 * KBS && B[t] != 0.0) bw = max(bw, q < KBS ? KBS - q : q - KBS);
+    }
+    return bw;
+}
+
+// Elimination with the narrowest window that holds the matrix's non-zeros
+// (bandwidth bw): the same operations on the non-zero entries, fewer on
+// structural zeros, so the same values.
+template <int KBS, bool LDS_, bool ONE_WAVE = false>
+__device__ __forceinline__ void eps_band_solve_bw(int bw, int first, int n, double* __restrict__ B,
+                                                  double* __restrict__ y, double* __restrict__ eps,
+                                                  double* __restrict__ xb) {
+    if (bw <= 2) eps_band_solve_impl<2, KBS, LDS_, ONE_WAVE>(first, n, B, y, eps, xb);
+    else if (bw == 3) eps_band_solve_impl<3, KBS, LDS_, ONE_WAVE>(first, n, B, y, eps, xb);
+    else if (bw == 4) eps_band_solve_impl<4, KBS, LDS_, ONE_WAVE>(first, n, B, y, eps, xb);
+    else eps_band_solve_impl<KBS, KBS, LDS_, ONE_WAVE>(first, n, B, y, eps, xb);
+}
+
+// One system per 64-lane workgroup: stage the band in LDS when it fits,
+// measure its bandwidth on the way, and solve with the matching window.
+// bw_sh: one int of LDS.
 template <int KB>
 __device__ __forceinline__ void eps_band_solve_body(int first, int n, double* __restrict__ G, double* __restrict__ ywork,
                                                     double* __restrict__ eps, int lds_rows, double* sm,
-                                                    double* xb) {
+                                                    double* xb, int* bw_sh) {
     constexpr int W = 3 * KB + 1;
+    const int nw = n * W;
+    if (threadIdx.x == 0) *bw_sh = 0;
+    __syncthreads();
+    int bw = 0;
     if (n <= lds_rows) {
-        const int nw = n * W;
 #pragma unroll 8
-        for (int t = threadIdx.x; t < nw; t += 64) sm[t] = G[t];
-        __syncthreads();
-        eps_band_solve_impl<KB, true>(first, n, sm, sm + (long long)nw, eps, xb);
+        for (int t = threadIdx.x; t < nw; t += 64) {
+            const double v = G[t];
+            sm[t] = v;
+            const int q = t % W;
+            if (q <= 2 * KB && v != 0.0) bw = max(bw, q < KB ? KB - q : q - KB);
+        }
     } else {
-        eps_band_solve_impl<KB, false>(first, n, G, ywork + first, eps, xb);
+        bw = band_width_rows<KB>(G, threadIdx.x, nw, 64);
     }
+    atomicMax(bw_sh, bw);
+    __syncthreads();
+    bw = *bw_sh;
+    if (n <= lds_rows) eps_band_solve_bw<KB, true>(bw, first, n, sm, sm + (long long)nw, eps, xb);
+    else eps_band_solve_bw<KB, false>(bw, first, n, G, ywork + first, eps, xb);
 }
 
 template <int KB>
@@ -394,10 +461,11 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
                                                        int lds_rows) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
+    __shared__ int bw;
     const int sys = blockIdx.x;
     coupling_priority();
     if (outside[sys]) return;
-    eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm, xb);
+    eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm, xb, &bw);
 }
 
 template <int KB>
@@ -405,11 +473,12 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
                                                              const CaseSys* __restrict__ sys, int lds_rows) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
+    __shared__ int bw;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
     if (c.eps.outside[q.k]) return;
     eps_band_solve_body<KB>(c.eps.first[q.k], c.eps.n[q.k], c.eps.band + c.eps.off[q.k], c.eps.y, c.m.eps, lds_rows,
-                            sm, xb);
+                            sm, xb, &bw);
 }
 
 // ---------------------------------------------------------------- fused recompute
@@ -432,14 +501,18 @@ constexpr int kFusedMaxMarkers = 256;
 template <int KB>
 __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridScalars& g, int first, int n,
                                                int* __restrict__ outside, double* __restrict__ sm,
-                                               double* __restrict__ xb, int* __restrict__ flag) {
+                                               double* __restrict__ xb, int* __restrict__ flag,
+                                               int* __restrict__ bw_sh) {
     constexpr int W = 3 * KB + 1;
     double* P = sm;                // 2n positions
     double* dsl = sm + 2 * n;      // n spacings
     double* Bl = dsl + n;          // n * W band
     double* yl = Bl + n * W;       // n right-hand side
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0) *flag = 0;
+    if (tid == 0) {
+        *flag = 0;
+        *bw_sh = 0;
+    }
     for (int k = tid; k < n; k += nt) {
         find_support_one(m, g, 1, first + k);
         P[2 * k] = m.pos[2 * (first + k)];
@@ -464,6 +537,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     __syncthreads();
     // band entries (as eps_band_entry, reading the staged positions and ds)
     const double Dx = g.Dx;
+    int bw = 0;
     for (int t = tid; t < n * W; t += nt) {
         const int r = t / W, q = t - r * W;
         const int c = r - KB + q;
@@ -480,7 +554,9 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
             a = a * dsl[c];
         }
         Bl[t] = a;
+        if (a != 0.0) bw = max(bw, q < KB ? KB - q : q - KB);
     }
+    atomicMax(bw_sh, bw);
     // pairs outside the band that could couple (as eps_far_pair)
     const double lim = 3.01 * g.Dx;
     for (int t = tid; t < n * n; t += nt) {
@@ -492,7 +568,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     const int far = *flag;
     if (tid == 0) *outside = far;
     if (far || tid >= 64) return;
-    eps_band_solve_impl<KB, true, true>(first, n, Bl, yl, m.eps, xb);
+    eps_band_solve_bw<KB, true, true>(*bw_sh, first, n, Bl, yl, m.eps, xb);
 }
 
 constexpr int kFusedThreads = 1024;
@@ -502,10 +578,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_recompute_body_batch(const Ca
                                                               const CaseSys* __restrict__ sys) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
-    __shared__ int flag;
+    __shared__ int flag, bw;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
-    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag);
+    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag, &bw);
 }
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at

@@ -6,7 +6,11 @@
 #include "lu.h"
 
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 namespace life {
@@ -107,11 +111,21 @@ int guarded(F&& f) {
 
 }  // namespace
 
+void lbm_join(life_grid* g) {
+    if (!g->tail_pending) return;
+    LIFE_HIP(hipStreamWaitEvent(g->stream, g->ev_tail, 0));
+    g->tail_pending = false;
+}
+
 void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (g->explicit_state) throw std::runtime_error("early stream-collide needs the populations' own moments");
     const int src = g->cur, dst = 1 - g->cur;
-    LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
-    LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_ready, 0));
+    // populations final: the step's tail is already queued on stream2, or
+    // the step ran on the grid's stream
+    if (!g->tail_pending) {
+        LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
+        LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_ready, 0));
+    }
     hipStream_t s = g->stream2;
     if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
     if (g->prec == kFP64)
@@ -134,8 +148,14 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
     hipStream_t s = g->stream;
     const bool early = g->bulk_pending;
     if (early) {
-        LIFE_HIP(hipStreamWaitEvent(s, g->ev_bulk, 0));
+        // the rest of the step follows the bulk kernel on stream2, after
+        // the grid stream's work so far (the spread it reads)
+        LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
+        LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_ready, 0));
+        s = g->stream2;
         g->bulk_pending = false;
+    } else {
+        lbm_join(g);
     }
     if (g->prec == kFP64) {
         const double* fs = static_cast<const double*>(g->f[src]);
@@ -161,12 +181,17 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
         launch_bc<float>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
     }
     LIFE_HIP(hipGetLastError());
+    if (early) {
+        LIFE_HIP(hipEventRecord(g->ev_tail, s));
+        g->tail_pending = true;
+    }
     g->cur = dst;
     g->explicit_state = false;
 }
 
 void ibm_interpolate(life_grid* g) {
     if (g->md.n <= 0) return;
+    lbm_join(g);
     if (g->prec == kFP64) launch_interpolate<double>(static_cast<const double*>(g->f[g->cur]), g->md, g->g, g->stream);
     else launch_interpolate<float>(static_cast<const float*>(g->f[g->cur]), g->md, g->g, g->stream);
     LIFE_HIP(hipGetLastError());
@@ -291,6 +316,56 @@ int life_device_count(int* count) {
     });
 }
 
+namespace {
+// CUs kept free of the early bulk stream-collide (second stream) for the
+// coupling kernels that run beside it: without a reserve, their small
+// workgroups wait for stream-collide workgroups to retire.  Environment
+// LIFE_COUPLING_CUS (default 16; 0: no reserve).
+int coupling_cus() {
+    static const int n = [] {
+        const char* e = std::getenv("LIFE_COUPLING_CUS");
+        const int v = e ? std::atoi(e) : 16;
+        return v < 0 ? 0 : v;
+    }();
+    return n;
+}
+
+// The second stream, masked to all CUs but the reserve.  The reserved CUs
+// are spread over the 8 XCDs whether the mask bits number the CUs XCD by
+// XCD or round-robin over the XCDs: CU x*G + ((x + 8*k) mod G) for XCD x
+// and k = 0, 1, ... (G = CUs per XCD) falls on XCD x either way.
+// A CU-masked stream has a hardware queue of its own, so there is one per
+// device for the whole process, shared by every grid on it (never
+// destroyed).  Sharing cannot deadlock: a grid records the event its bulk
+// kernel waits for before it queues that wait.
+hipStream_t bulk_stream(int device) {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = streams.find(device);
+    if (it != streams.end()) return it->second;
+    int ncu = 0;
+    LIFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    const int reserve = coupling_cus() < ncu / 8 ? coupling_cus() : ncu / 8;  // at most G/8 per XCD
+    hipStream_t st = nullptr;
+    if (reserve == 0 || ncu % 8 != 0) {
+        LIFE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    } else {
+        const int G = ncu / 8;
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+        for (int c = 0; c < ncu; c++) mask[(size_t)c / 32] |= 1u << (c % 32);
+        for (int r = 0; r < reserve; r++) {
+            const int x = r % 8, k = r / 8;
+            const int c = x * G + (x + 8 * k) % G;
+            mask[(size_t)c / 32] &= ~(1u << (c % 32));
+        }
+        LIFE_HIP(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
+    }
+    streams[device] = st;
+    return st;
+}
+}  // namespace
+
 int life_grid_create(const life_params* p, int precision, int device, life_grid** out) {
     *out = nullptr;
     life_grid* g = new life_grid();
@@ -307,9 +382,10 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         LIFE_HIP(hipEventCreate(&g->ev_b));
         LIFE_HIP(hipEventCreate(&g->ev_k0));
         LIFE_HIP(hipEventCreate(&g->ev_k1));
-        LIFE_HIP(hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking));
+        g->stream2 = bulk_stream(device);
         LIFE_HIP(hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming));
         LIFE_HIP(hipEventCreateWithFlags(&g->ev_bulk, hipEventDisableTiming));
+        LIFE_HIP(hipEventCreateWithFlags(&g->ev_tail, hipEventDisableTiming));
         g->p = *p;
         g->prec = precision;
         g->cs = derive_scalars(*p);
@@ -430,17 +506,17 @@ int life_grid_destroy(life_grid* g) {
     return guarded([&] {
         (void)hipSetDevice(g->device);
         if (g->stream) (void)hipStreamSynchronize(g->stream);
+        if (g->ev_bulk) (void)hipEventSynchronize(g->ev_bulk);  // this grid's last bulk kernel
+        if (g->ev_tail) (void)hipEventSynchronize(g->ev_tail);  // and step tail on stream2
         free_markers(g);
         for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm, (void*)g->bc,
                         (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA,
                         (void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
                         (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
-        if (g->stream2) (void)hipStreamSynchronize(g->stream2);
         if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
-        for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk})
+        for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk, g->ev_tail})
             if (e) (void)hipEventDestroy(e);
-        if (g->stream2) (void)hipStreamDestroy(g->stream2);
         if (g->stream) (void)hipStreamDestroy(g->stream);
         delete g;
     });
@@ -456,6 +532,7 @@ int life_grid_scalars(const life_grid* g, double out[10]) {
 int life_lbm_kernel(life_grid* g, int t) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(g->device));
+        lbm_join(g);
         lbm_step(g, t, nullptr, nullptr);
     });
 }
@@ -463,6 +540,7 @@ int life_lbm_kernel(life_grid* g, int t) {
 int life_grid_sync(life_grid* g) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(g->device));
+        lbm_join(g);
         LIFE_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -470,6 +548,7 @@ int life_grid_sync(life_grid* g) {
 int life_grid_download(life_grid* g, double* f_aos, double* rho, double* u, double* force_ibm) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(g->device));
+        lbm_join(g);
         const size_t N = (size_t)g->g.N;
         double *df = nullptr, *dr = nullptr, *du = nullptr, *db = nullptr;
         try {
@@ -501,6 +580,7 @@ int life_grid_upload(life_grid* g, const double* f_aos, const double* rho, const
     return guarded([&] {
         if (!f_aos || !rho || !u) throw std::runtime_error("life_grid_upload needs f, rho and u");
         LIFE_HIP(hipSetDevice(g->device));
+        lbm_join(g);
         const size_t N = (size_t)g->g.N;
         double* df = dalloc<double>(9 * N);
         double* db = nullptr;

@@ -94,10 +94,12 @@ struct life_grid {
     std::vector<int> eps_first_h, eps_n_h;
     // timing
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
-    // early bulk stream-collide of the next step on a second stream
+    // early bulk stream-collide of the next step on a second stream (one
+    // CU-masked stream per device, shared by all grids; not owned)
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_ready = nullptr, ev_bulk = nullptr;
-    bool bulk_pending = false;
+    hipEvent_t ev_ready = nullptr, ev_bulk = nullptr, ev_tail = nullptr;
+    bool bulk_pending = false;  // a bulk kernel of the next step is queued on stream2
+    bool tail_pending = false;  // the last step's tail ran on stream2 (stream not yet joined)
 
     void* f_cur() { return f[cur]; }
 };
@@ -114,6 +116,13 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end);
 // boundaries) after the bulk kernel.  Not for the first step after
 // creation or upload (explicit rho/u).
 void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end);
+// After lbm_bulk_early, lbm_step queues the rest of the step (outlet speed,
+// support sites, boundaries) on the second stream behind the bulk kernel,
+// once the first stream's work so far (the spread) is done, so that the
+// next bulk kernel follows it there with no cross-stream wait; lbm_join
+// then makes the grid's stream wait for that work before it reads the
+// lattice (interpolation, downloads, the end of a call).
+void lbm_join(life_grid* g);
 void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);
