@@ -368,11 +368,25 @@ def main():
     traffic = read_traffic(kernel, wcfg["workload"])
     print("bench: %d steps in %.3f s wall (%.3f ms device), K1 avg %.3f ms" %
           (args.steps, wall, ms_dev, k_avg_s * 1e3), file=sys.stderr, flush=True)
-
     accuracy = None
+    u32 = None
     if not args.no_accuracy and args.precision == "f32":
         _, _, u32, _ = case.fields()
-        case.close()
+    # the same kernel with nothing beside it: a few steps without the
+    # overlap of the next step's stream-collide and the coupling (after the
+    # fields for the accuracy check were taken)
+    alone_steps = min(args.steps, 20)
+    saved = os.environ.get("LIFE_OVERLAP")
+    os.environ["LIFE_OVERLAP"] = "0"
+    _, ms_alone = case.step_timed(alone_steps)
+    if saved is None:
+        del os.environ["LIFE_OVERLAP"]
+    else:
+        os.environ["LIFE_OVERLAP"] = saved
+    k_alone_s = ms_alone / 1e3 / alone_steps
+
+    case.close()
+    if u32 is not None:
         ref = Case(p, geom, precision=lib.LIFE_FP64)
         ref.step(args.warmup + args.steps)
         _, _, u64, _ = ref.fields()
@@ -380,8 +394,6 @@ def main():
         accuracy = {"u_l2_rel_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
                     "steps": args.warmup + args.steps}
         del u32, u64
-    else:
-        case.close()
 
     cpu = None
     if not args.no_cpu:
@@ -409,7 +421,10 @@ def main():
         "config": dict(wcfg, precision=("fp32_shifted" if args.precision == "f32" else "fp64")),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                     "kernel_ms": k_avg_s * 1e3, "kernel_mlups": N / k_avg_s / 1e6},
+                     "kernel_ms": k_avg_s * 1e3, "kernel_mlups": N / k_avg_s / 1e6,
+                     "alone": {"steps": alone_steps, "kernel_ms": k_alone_s * 1e3,
+                               "achieved": algo_bytes / k_alone_s / 1e9,
+                               "frac": algo_bytes / k_alone_s / 1e9 / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }

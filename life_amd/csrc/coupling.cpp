@@ -343,15 +343,13 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
 // the spread of step t, as in the reference order.
 // LIFE_OVERLAP = 0 (off), 1 (launch right after this step's lattice
 // kernels, default; measured fastest) or 2 (after the first interpolation
-// of the coupling).
+// of the coupling).  Read at every step, so a caller can switch it between
+// calls.
 int overlap_mode() {
-    static const int mode = [] {
-        const char* e = std::getenv("LIFE_OVERLAP");
-        if (!e) return 1;
-        const int v = std::atoi(e);
-        return v >= 0 && v <= 2 ? v : 1;
-    }();
-    return mode;
+    const char* e = std::getenv("LIFE_OVERLAP");
+    if (!e) return 1;
+    const int v = std::atoi(e);
+    return v >= 0 && v <= 2 ? v : 1;
 }
 
 // One main-loop step; next_begin/next_end (may be null) are recorded
@@ -364,9 +362,11 @@ void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, bool more, 
     c->t++;
     lbm_step(c->g, c->t, ev_begin, ev_end);
     T.lap(c->g, 0);
-    const int mode = T.on ? 0 : overlap_mode();
+    // without immersed bodies there is nothing to overlap the early
+    // stream-collide with: the plain order avoids the cross-stream waits
+    const int mode = T.on || !c->hasIBM ? 0 : overlap_mode();
     EarlyBulk eb{more && mode != 0, next_begin, next_end};
-    if (mode == 1 || !c->hasIBM) launch_pending(c, eb);
+    if (mode == 1) launch_pending(c, eb);
     if (c->hasIBM) object_kernel(c, eb, mode);
     launch_pending(c, eb);
     T.t[5] += 1;

@@ -14,8 +14,19 @@ namespace life {
 // The single-case coupling kernels run beside the next step's bulk
 // stream-collide (launched early on another stream): their waves take the
 // highest issue priority on a SIMD they share with stream-collide waves,
-// since they are the critical path of the step.
-__device__ __forceinline__ void coupling_priority() { __builtin_amdgcn_s_setprio(3); }
+// since they are usually the critical path of the step.  The level (0..3)
+// is set per device by set_coupling_priority (LIFE_COUPLING_PRIO, default 3).
+__device__ int g_coupling_prio = 3;
+__device__ __forceinline__ void coupling_priority() {
+    const int p = g_coupling_prio;
+    if (p >= 3) __builtin_amdgcn_s_setprio(3);
+    else if (p == 2) __builtin_amdgcn_s_setprio(2);
+    else if (p == 1) __builtin_amdgcn_s_setprio(1);
+}
+
+hipError_t set_coupling_priority(int level) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_coupling_prio), &level, sizeof(int));
+}
 
 // IBMNodeClass::findSupport (IBMNode.cpp:139-179): lattice sites within 1.5
 // spacings of the marker in both directions, scanned i-major over the 5x5

@@ -156,6 +156,9 @@ template <typename T>
 void launch_bc_batch(bool cm, const CaseDev* cases, const int* list, int n, int max_nbc, hipStream_t s);
 template <typename T>
 void launch_convective_batch(const CaseDev* cases, const int* list, int n, int max_ny, bool ordered, hipStream_t s);
+// Wave priority (s_setprio level 0..3) of the single-case coupling kernels
+// on the current device.
+hipError_t set_coupling_priority(int level);
 // packed marker staging: in = [x, y, vx, vy] per marker, out = [Fx, Fy, eps]
 void launch_markers_in_batch(const CaseDev* cases, const int* list, int n, int max_markers, const double* stage,
                              hipStream_t s);

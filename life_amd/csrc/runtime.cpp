@@ -318,13 +318,15 @@ int life_device_count(int* count) {
 
 namespace {
 // CUs kept free of the early bulk stream-collide (second stream) for the
-// coupling kernels that run beside it: without a reserve, their small
-// workgroups wait for stream-collide workgroups to retire.  Environment
-// LIFE_COUPLING_CUS (default 16; 0: no reserve).
+// coupling kernels that run beside it (environment LIFE_COUPLING_CUS,
+// default 0: no reserve).  A reserve lets the small coupling workgroups
+// start without waiting for stream-collide workgroups to retire; it paid
+// while the coupling chain was the critical path, and costs stream-collide
+// time once the coupling fits beside it.
 int coupling_cus() {
     static const int n = [] {
         const char* e = std::getenv("LIFE_COUPLING_CUS");
-        const int v = e ? std::atoi(e) : 16;
+        const int v = e ? std::atoi(e) : 0;
         return v < 0 ? 0 : v;
     }();
     return n;
@@ -338,6 +340,20 @@ int coupling_cus() {
 // device for the whole process, shared by every grid on it (never
 // destroyed).  Sharing cannot deadlock: a grid records the event its bulk
 // kernel waits for before it queues that wait.
+// Wave priority of the coupling kernels (LIFE_COUPLING_PRIO, 0..3, default
+// 3), set once per device.
+void init_coupling_priority(int device) {
+    static std::mutex mu;
+    static std::map<int, bool> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[device]) return;
+    const char* e = std::getenv("LIFE_COUPLING_PRIO");
+    int v = e ? std::atoi(e) : 3;
+    v = v < 0 ? 0 : (v > 3 ? 3 : v);
+    LIFE_HIP(set_coupling_priority(v));
+    done[device] = true;
+}
+
 hipStream_t bulk_stream(int device) {
     static std::mutex mu;
     static std::map<int, hipStream_t> streams;
@@ -383,6 +399,7 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         LIFE_HIP(hipEventCreate(&g->ev_k0));
         LIFE_HIP(hipEventCreate(&g->ev_k1));
         g->stream2 = bulk_stream(device);
+        init_coupling_priority(device);
         LIFE_HIP(hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming));
         LIFE_HIP(hipEventCreateWithFlags(&g->ev_bulk, hipEventDisableTiming));
         LIFE_HIP(hipEventCreateWithFlags(&g->ev_tail, hipEventDisableTiming));
