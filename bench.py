@@ -229,6 +229,20 @@ def bench_serve(args):
                 "throughput_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
                 "p99_ratio": plain["p99_s"] / served["p99_s"],
                 "accuracy": serve_accuracy(pre, pinfo, pout, args)}
+    capacity = None
+    if args.serve_capacity:
+        # the same cases all queued at once: the engine's own throughput,
+        # not bounded by the arrival rate
+        burst = [dict(r, arrival=0.0) for r in stream]
+        srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
+        t0 = time.perf_counter()
+        bids = srv.run(burst)
+        bwall = time.perf_counter() - t0
+        capacity, _ = serve_summary(srv, bids, bwall)
+        srv.close()
+        if base is not None:
+            capacity["ratio_vs_plain_fp64"] = capacity["cases_per_hour"] / base["plain_fp64_one_at_a_time"]["cases_per_hour"]
+        print("bench: burst served in %.1f s" % bwall, file=sys.stderr, flush=True)
     out = {
         "metric": "FSI cases/hour (serving engine, continuous batching)",
         "value": full["cases_per_hour"],
@@ -245,6 +259,7 @@ def bench_serve(args):
         "config": wl,
         "serving": full,
         "vs_plain_fp64_one_at_a_time": base,
+        "capacity_all_queued_at_once": capacity,
     }
     print(json.dumps(out))
 
@@ -331,6 +346,8 @@ def main():
     ap.add_argument("--baseline-cases", type=int, default=32)
     ap.add_argument("--accuracy-cases", type=int, default=8)
     ap.add_argument("--accuracy-steps", type=int, default=300)
+    ap.add_argument("--serve-capacity", action="store_true",
+                    help="also run the same cases all queued at once (engine throughput without arrivals)")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
