@@ -156,6 +156,30 @@ int life_case_coupling_info(const life_case *c, double out[3]);
 /* ObjectsClass::writeTotalForces quantities (Objects.cpp:517-534): Fx, Fy, Cx, Cy */
 int life_case_total_force(life_case *c, double out[4]);
 
+/* Result files in the reference's formats, under a results directory laid
+ * out as the reference's Results/ folder.  Binary data is in the host's byte
+ * order (little-endian here), as the reference writes it on such a host. */
+/* GridClass::writeRestart + ObjectsClass::writeRestart (Grid.cpp:1163-1229,
+ * Objects.cpp:1206-1386): <results>/Restart/Fluid.restart, IBM.restart (bodies)
+ * and FEM.restart (flexible bodies), each written to a .temp file first. */
+int life_case_write_restart(life_case *c, const char *results);
+/* GridClass::readRestart + ObjectsClass::readRestart + restartCleanup
+ * (Grid.cpp:1072-1160, Objects.cpp:979-1204, 1387-1586): restores the lattice
+ * (the next step uses the stored rho/u), the time step, the markers and the
+ * FEM state, recomputes supports, spacing and epsilon, and trims the force and
+ * tip logs to the steps before the restart. */
+int life_case_read_restart(life_case *c, const char *results);
+/* GridClass::writeVTK + ObjectsClass::writeVTK (Grid.cpp:790-913,
+ * Objects.cpp:711-864): <results>/VTK/Fluid.<t>.vti, IBM.<t>.vtp and, with
+ * fem = 1 (the reference's VTK_FEM), FEM.<t>.vtp. */
+int life_case_write_vtk(life_case *c, const char *results, int fem);
+/* ObjectsClass::writeTotalForces (Objects.cpp:491-539): appends a line to
+ * <results>/TotalForces.out (header first when the file is new). */
+int life_case_write_forces(life_case *c, const char *results);
+/* ObjectsClass::writeTips (Objects.cpp:542-594): appends to
+ * <results>/TipPositions.out and TipVelocities.out. */
+int life_case_write_tips(life_case *c, const char *results);
+
 /* ---------------------------------------------------------------- serving
  *
  * Continuous batching of independent cases on one device.  The reference

@@ -172,6 +172,22 @@ class Case:
         L.check(L.load().life_case_get_fem(self.h, int(b), L.dptr(U), L.dptr(Ud), L.dptr(Udd)))
         return U, Ud, Udd
 
+    # result files in the reference's formats (Results/ layout)
+    def write_restart(self, results):
+        L.check(L.load().life_case_write_restart(self.h, str(results).encode()))
+
+    def read_restart(self, results):
+        L.check(L.load().life_case_read_restart(self.h, str(results).encode()))
+
+    def write_vtk(self, results, fem=False):
+        L.check(L.load().life_case_write_vtk(self.h, str(results).encode(), 1 if fem else 0))
+
+    def write_forces(self, results):
+        L.check(L.load().life_case_write_forces(self.h, str(results).encode()))
+
+    def write_tips(self, results):
+        L.check(L.load().life_case_write_tips(self.h, str(results).encode()))
+
     def coupling(self):
         out = np.zeros(3)
         L.check(L.load().life_case_coupling_info(self.h, L.dptr(out)))

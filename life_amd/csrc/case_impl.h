@@ -128,5 +128,9 @@ void case_fem(life_case* c);
 inline bool case_subiterate(const life_case* c) { return c->subIt < kMaxSubIt && c->subRes > c->p.subTol; }
 // computeEpsilon for markers [first, first+n) by the dense path
 void case_solve_eps_dense(life_case* c, int first, int n);
+// After the marker arrays were restored from a restart file
+// (ObjectsClass::readRestart, Objects.cpp:1186-1198): upload them, find the
+// supports and spacing of every marker and recompute epsilon as at t > 0.
+void case_restart_objects(life_case* c);
 
 }  // namespace life

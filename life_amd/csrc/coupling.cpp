@@ -269,6 +269,24 @@ void life::case_solve_eps_dense(life_case* c, int first, int n) {
                             c->g->stream));
 }
 
+void life::case_restart_objects(life_case* c) {
+    life_grid* g = c->g;
+    LIFE_HIP(hipSetDevice(g->device));
+    LIFE_HIP(hipEventSynchronize(c->ev_upload));
+    const size_t b2 = sizeof(double) * 2 * c->nm;
+    LIFE_HIP(hipMemcpyAsync(g->md.pos, c->pos.data(), b2, hipMemcpyHostToDevice, g->stream));
+    LIFE_HIP(hipMemcpyAsync(g->md.vel, c->vel.data(), b2, hipMemcpyHostToDevice, g->stream));
+    LIFE_HIP(hipMemcpyAsync(g->md.force, c->force.data(), b2, hipMemcpyHostToDevice, g->stream));
+    ibm_find_support(g, 0);
+    compute_epsilon(c, false);
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    finish_epsilon(c);
+    LIFE_HIP(hipMemcpyAsync(c->eps.data(), g->md.eps, sizeof(double) * c->nm, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->ds.data(), g->md.ds, sizeof(double) * c->nm, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    check_flags(g);
+}
+
 namespace {
 
 // ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.

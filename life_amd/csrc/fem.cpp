@@ -146,6 +146,33 @@ FemBody::FemBody(const FemScalars& s, const double pos[2], const double geom[2],
     vec_.assign(ndof_, 0.0);
 }
 
+void FemBody::node_state(int n, double out[6]) const {
+    const Node& d = node_[n];
+    out[0] = d.pos0[0];
+    out[1] = d.pos0[1];
+    out[2] = d.pos[0];
+    out[3] = d.pos[1];
+    out[4] = d.angle0;
+    out[5] = d.angle;
+}
+
+void FemBody::set_element_state(int e, double L, double angle) {
+    Elem& el = el_[e];
+    el.L = L;
+    el.angle = angle;
+    transform(el);
+}
+
+void FemBody::set_node_state(int n, const double in[6]) {
+    Node& d = node_[n];
+    d.pos0[0] = in[0];
+    d.pos0[1] = in[1];
+    d.pos[0] = in[2];
+    d.pos[1] = in[3];
+    d.angle0 = in[4];
+    d.angle = in[5];
+}
+
 void FemBody::transform(Elem& e) {
     const double ca = std::cos(e.angle), sa = std::sin(e.angle);
     e.T[0][0] = e.T[1][1] = e.T[3][3] = e.T[4][4] = ca;

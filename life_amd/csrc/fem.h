@@ -59,6 +59,21 @@ class FemBody {
 
     std::vector<double> U, Ud, Udd, Un, Unm1, Unm2, Udn, Uddn, Ukm1, Rk, Rkm1, R, F, delU;
 
+    // Element and node state of the restart file (FEM.restart, Objects.cpp:1206-1386):
+    // per element L and angle; per node pos0[2], pos[2], angle0, angle.
+    int num_elements() const { return (int)el_.size(); }
+    int num_nodes() const { return (int)node_.size(); }
+    void element_state(int e, double& L, double& angle) const {
+        L = el_[e].L;
+        angle = el_[e].angle;
+    }
+    void node_state(int n, double out[6]) const;
+    // Restore them as ObjectsClass::readRestart does (Objects.cpp:1164-1201):
+    // the element transforms follow the angles; the local matrices depend on
+    // L0 only and are unchanged.
+    void set_element_state(int e, double L, double angle);
+    void set_node_state(int n, const double in[6]);
+
   private:
     struct Node {
         double pos0[2], pos[2], angle0, angle;

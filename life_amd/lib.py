@@ -95,6 +95,11 @@ SIGNATURES = {
     "life_case_get_fem": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "life_case_coupling_info": (C.c_int, [_vp, _dp]),
     "life_case_total_force": (C.c_int, [_vp, _dp]),
+    "life_case_write_restart": (C.c_int, [_vp, C.c_char_p]),
+    "life_case_read_restart": (C.c_int, [_vp, C.c_char_p]),
+    "life_case_write_vtk": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "life_case_write_forces": (C.c_int, [_vp, C.c_char_p]),
+    "life_case_write_tips": (C.c_int, [_vp, C.c_char_p]),
     "life_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "life_server_destroy": (C.c_int, [_vp]),
     "life_server_submit": (C.c_int, [_vp, C.POINTER(LifeParams), C.c_char_p, C.c_int, C.c_int, _ip]),
