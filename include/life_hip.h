@@ -58,6 +58,9 @@ typedef struct life_params {
     int uni_epsilon;         /* UNI_EPSILON (params.h:35) */
     double initial_deflect;  /* INITIAL_DEFLECT; 0: undefined (params.h:37) */
     double ref_nu, ref_rho, ref_P, ref_L, ref_U;
+    double womersley;        /* WOMERSLEY number of the oscillating pressure gradient dpdx/dpdy
+                                (params.h:34, Grid.cpp:52-61); 0: undefined.  Needs
+                                gravityX = gravityY = 0 (the force is then uniform). */
 } life_params;
 
 typedef struct life_grid life_grid;

@@ -57,6 +57,7 @@ class CaseParams:
     ref_P: float = 0.0
     ref_L: float = 1.0
     ref_U: float = 1.0
+    womersley: float = 0.0      # WOMERSLEY, 0: undefined
 
     def replace(self, **kw):
         return dataclasses.replace(self, **kw)
@@ -89,6 +90,54 @@ def lid_driven_cavity(N=256, tau=0.6, u_lid=0.05, central_moments=0):
                      profile=UNIFORM, uxInlet_p=u_lid, uyInlet_p=0.0, central_moments=central_moments,
                      ref_L=float(N - 1), ref_U=u_lid)
     return p, ""
+
+
+def womersley_channel(Nx=4, Ny=33, tau=0.8, Wo=4.0, G=1e-5, central_moments=0):
+    """Oscillating pressure-gradient channel (WOMERSLEY, Grid.cpp:52-61):
+    periodic in x, regularised no-slip walls on the first and last rows,
+    body force G cos(2 pi t / T) with period T = H^2 pi / (2 Wo^2 nu),
+    H = Ny - 1, so Wo = (H / 2) sqrt(omega / nu)."""
+    return lattice_case(Nx, Ny, tau, wall_left=FLUID, wall_right=FLUID, wall_bottom=WALL, wall_top=WALL,
+                        profile=UNIFORM, uxInlet_p=0.0, dpdx=G, womersley=Wo, central_moments=central_moments,
+                        ref_L=float(Ny - 1), ref_U=G)
+
+
+def womersley_cylinder(Nx=96, Ny=49, D=10.0, tau=0.8, Wo=4.0, G=2e-5, central_moments=1):
+    """The Womersley channel with a rigid CIRCLE on the centre line (the
+    oscillating force acting beside the immersed-boundary force)."""
+    p = womersley_channel(Nx=Nx, Ny=Ny, tau=tau, Wo=Wo, G=G, central_moments=central_moments)
+    geom = "CIRCLE 1 0 %.17g %.17g 0.0 0.0 %.17g\n" % (Nx / 3.0, (Ny - 1) / 2.0, D / 2.0)
+    return p.replace(ref_L=D), geom
+
+
+def pelskin(resFactor=3):
+    """The reference's PELskin example (examples/PELskin/params.h and
+    input/geometry.config) in its own physical units: a periodic channel of
+    height 0.06 m driven by the oscillating pressure gradient dpdx = 4550
+    Pa/m at Womersley number 7.52, BGK, with ten clamped flexible filaments
+    (20 elements each) standing on the bottom wall, uniform epsilon system,
+    subTol 1e-4.  Nx = 245 resFactor, Ny = 30 resFactor + 1, tStep =
+    (0.001 / 3) / resFactor^2 (omega follows from it)."""
+    p = CaseParams(Nx=resFactor * (5 * 9 + 10 * 20), Ny=resFactor * (10 * 3) + 1, height_p=0.06, rho_p=1200.0,
+                   nu_p=0.0001, dpdx=4550.0, wall_left=FLUID, wall_right=FLUID, wall_bottom=WALL, wall_top=WALL,
+                   profile=UNIFORM, uxInlet_p=0.0, subTol=1e-4, tStep=(0.001 / 3.0) / (resFactor * resFactor),
+                   uni_epsilon=1, womersley=7.52, ref_nu=0.0001, ref_rho=1200.0, ref_L=0.02, ref_U=0.06)
+    geom = "FILAMENT 10 0 0.2 0.0 0.01 0.0 0.02 0.001 90.0 FLEXIBLE 20 CLAMPED 1200.0 1.2e6\n"
+    return p, geom
+
+
+def womersley_profile(y, t, H, nu, Wo, G):
+    """Planar Womersley flow between walls at y = 0 and y = H driven by the
+    body force G cos(omega t): u = Re{G / (i omega) (1 - cosh(k (y - H/2)) /
+    cosh(k H / 2)) e^(i omega t)}, k = sqrt(i omega / nu)."""
+    import cmath
+    omega = 4.0 * Wo * Wo * nu / (H * H)
+    k = cmath.sqrt(1j * omega / nu)
+    out = []
+    for yy in y:
+        U = G / (1j * omega) * (1.0 - cmath.cosh(k * (yy - H / 2.0)) / cmath.cosh(k * H / 2.0))
+        out.append((U * cmath.exp(1j * omega * t)).real)
+    return out
 
 
 def cylinder_channel(Nx=1024, Ny=256, D=40.0, centre=(200.0, 128.0), u_max=0.04, Re=100.0,

@@ -153,9 +153,12 @@ LIFE_DF void support_push32_site(const float* __restrict__ src, float* __restric
     float gs[kNV], go[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; v++) gs[v] = src[v * S + id];
+    // moments as stored at the end of the previous step (its force_xy),
+    // forcing of this step
+    const float Fxm = (float)(g.Fxm + fibm[id]), Fym = (float)(g.Fym + fibm[S + id]);
     const float Fx = (float)(g.Fx + fibm[id]), Fy = (float)(g.Fy + fibm[S + id]);
     float drho, ux, uy;
-    moments32(gs, Fx, Fy, drho, ux, uy);
+    moments32(gs, Fxm, Fym, drho, ux, uy);
     collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
     push_site32(dst, g, id, go);
 }

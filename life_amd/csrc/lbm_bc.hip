@@ -95,7 +95,7 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
     } else {
         double fp[kNV];
         load_site(src, S, id, fp);
-        moments64(fp, g.Fx + fibm[id], g.Fy + fibm[S + id], rn, uxn, uyn);
+        moments64(fp, g.Fxm + fibm[id], g.Fym + fibm[S + id], rn, uxn, uyn);
     }
     const long long id1 = (long long)(b.i + b.nx) * Ny + (j + b.ny);
     const long long id2 = (long long)(b.i + 2 * b.nx) * Ny + (j + 2 * b.ny);
@@ -190,7 +190,7 @@ __device__ inline void convective_column(const T* __restrict__ src, const GridSc
     } else {
         double fs[kNV], r;
         load_site(src, S, id, fs);
-        moments64(fs, g.Fx + fibm[id], g.Fy + fibm[S + id], r, ux, uy);
+        moments64(fs, g.Fxm + fibm[id], g.Fym + fibm[S + id], r, ux, uy);
     }
     ucol[(c * 2 + 0) * (long long)Ny + j] = ux;
     ucol[(c * 2 + 1) * (long long)Ny + j] = uy;

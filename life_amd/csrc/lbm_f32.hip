@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_push32(const float* __restrict__ src, f
         Fx = (float)(g.Fx + fibm[id]);
         Fy = (float)(g.Fy + fibm[S + id]);
     } else {
-        moments32(gs, Fx, Fy, drho, ux, uy);
+        moments32(gs, (float)g.Fxm, (float)g.Fym, drho, ux, uy);
     }
     collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
     const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void push32_lds_block(SP __restrict__ src, DP __restr
     for (int v = 0; v < kNV; v++) gs[v] = valid ? __builtin_nontemporal_load(&src[v * S + idc]) : 0.0f;
     float drho, ux, uy;
     const float Fx = (float)g.Fx, Fy = (float)g.Fy;
-    moments32(gs, Fx, Fy, drho, ux, uy);
+    moments32(gs, (float)g.Fxm, (float)g.Fym, drho, ux, uy);
     collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
     push_store_realigned<float, NTS, BS>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, go, valid, st);
 }
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_push32x4(const float* __restrict__ src,
 #pragma unroll
         for (int v = 0; v < kNV; v++) gs[v] = s == 0 ? in[v].x : (s == 1 ? in[v].y : (s == 2 ? in[v].z : in[v].w));
         float drho, ux, uy;
-        moments32(gs, Fx, Fy, drho, ux, uy);
+        moments32(gs, (float)g.Fxm, (float)g.Fym, drho, ux, uy);
         collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
 #pragma unroll
         for (int v = 0; v < kNV; v++) o[v][s] = go[v];

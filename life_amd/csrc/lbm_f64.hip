@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
         Fx = g.Fx + fibm[id];
         Fy = g.Fy + fibm[S + id];
     } else {
-        moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+        moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
     }
     collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
     const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
@@ -75,7 +75,7 @@ __device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restr
 #pragma unroll
     for (int v = 0; v < kNV; v++) fs[v] = valid ? __builtin_nontemporal_load(&src[v * S + idc]) : 1.0;
     double rho, ux, uy;
-    moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+    moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
     collide64<CM>(g, fs, rho, ux, uy, g.Fx0, g.Fy0, fo);
     push_store_realigned<double, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
 }
@@ -137,9 +137,12 @@ __device__ __forceinline__ void support_push64_site(const double* __restrict__ s
     const long long S = g.S;
     double fs[kNV], fo[kNV];
     load_site(src, S, id, fs);
+    // moments as stored at the end of the previous step (its force_xy),
+    // forcing of this step
+    const double Fxm = g.Fxm + fibm[id], Fym = g.Fym + fibm[S + id];
     const double Fx = g.Fx + fibm[id], Fy = g.Fy + fibm[S + id];
     double rho, ux, uy;
-    moments64(fs, Fx, Fy, rho, ux, uy);
+    moments64(fs, Fxm, Fym, rho, ux, uy);
     collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
     push_site64(dst, g, id, fo);
 }

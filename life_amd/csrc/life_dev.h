@@ -48,6 +48,8 @@ struct GridScalars {
     double cs2, cs4;        // c_s*c_s and c_s*c_s*c_s*c_s with the reference's rounding
     double Fx, Fy;          // force_xy (uniform)
     double Fx0, Fy0;        // force_xy + 0.0 (force_xy + force_ibm at non-support sites)
+    double Fxm, Fym;        // force_xy of the previous step: the moments of the incoming populations
+                            // (equal to Fx, Fy unless the forcing varies in time, WOMERSLEY)
     double Dx, velScale;    // Dx and Dt/Dx (IBMNode.cpp:54)
 };
 

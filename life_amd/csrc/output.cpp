@@ -157,6 +157,8 @@ void read_fluid_restart(life_case* c, const std::string& dir) {
             in.read(reinterpret_cast<char*>(&s.f[9 * id]), 9 * sizeof(double));
             if (!in) throw std::runtime_error("truncated Fluid.restart");
         }
+    c->g->g = step_scalars(*c->g, t);  // force_xy of the restart step (WOMERSLEY)
+    c->g->t_step = t;
     if (life_grid_upload(c->g, s.f.data(), s.rho.data(), s.u.data(), s.fibm.data()))
         throw std::runtime_error(life_last_error());
     c->t = t;

@@ -111,6 +111,40 @@ int guarded(F&& f) {
 
 }  // namespace
 
+void body_force(const life_grid& g, int t, double F[2]) {
+    const life_params& p = g.p;
+    const CaseScalars& cs = g.cs;
+    const double DxDt = cs.Dx * cs.Dt;
+    const double rho_init = 1.0;  // the initial density the reference multiplies by (Grid.cpp:1042)
+    double wave = 1.0;
+    if (p.womersley != 0.0 && t > 0) {
+        // Grid.cpp:59-60; the density factor only meets gravity, which is 0 here
+        const double period = ((p.height_p * p.height_p) * M_PI) / (2.0 * (p.womersley * p.womersley) * p.nu_p);
+        wave = std::cos(2.0 * M_PI * t * cs.Dt / period);
+        F[0] = (rho_init * cs.Drho * p.gravityX + p.dpdx * wave) * (DxDt * DxDt) / cs.Dm;
+        F[1] = (rho_init * cs.Drho * p.gravityY + p.dpdy * wave) * (DxDt * DxDt) / cs.Dm;
+        return;
+    }
+    // Grid.cpp:1042-1043
+    F[0] = (rho_init * cs.Drho * p.gravityX + p.dpdx) * (DxDt * DxDt) / cs.Dm;
+    F[1] = (rho_init * cs.Drho * p.gravityY + p.dpdy) * (DxDt * DxDt) / cs.Dm;
+}
+
+GridScalars step_scalars(const life_grid& g, int t) {
+    GridScalars s = g.g;
+    if (g.p.womersley == 0.0) return s;
+    double F[2], Fm[2];
+    body_force(g, t, F);
+    body_force(g, t - 1, Fm);
+    s.Fx = F[0];
+    s.Fy = F[1];
+    s.Fx0 = F[0] + 0.0;
+    s.Fy0 = F[1] + 0.0;
+    s.Fxm = Fm[0];
+    s.Fym = Fm[1];
+    return s;
+}
+
 void lbm_join(life_grid* g) {
     if (!g->tail_pending) return;
     LIFE_HIP(hipStreamWaitEvent(g->stream, g->ev_tail, 0));
@@ -128,11 +162,12 @@ void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
     }
     hipStream_t s = g->stream2;
     if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
+    const GridScalars next = step_scalars(*g, g->t_step + 1);  // the step this kernel belongs to
     if (g->prec == kFP64)
-        launch_push64(static_cast<const double*>(g->f[src]), static_cast<double*>(g->f[dst]), g->g, g->rho0, g->u0,
+        launch_push64(static_cast<const double*>(g->f[src]), static_cast<double*>(g->f[dst]), next, g->rho0, g->u0,
                       g->fibm, false, s);
     else
-        launch_push32(static_cast<const float*>(g->f[src]), static_cast<float*>(g->f[dst]), g->g, g->rho0, g->u0,
+        launch_push32(static_cast<const float*>(g->f[src]), static_cast<float*>(g->f[dst]), next, g->rho0, g->u0,
                       g->fibm, false, s);
     if (ev_end) LIFE_HIP(hipEventRecord(ev_end, s));
     LIFE_HIP(hipGetLastError());
@@ -141,6 +176,8 @@ void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
 }
 
 void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    g->g = step_scalars(*g, t);
+    g->t_step = t;
     const double ramp = ramp_coefficient(g->p, g->cs.Dt, t);
     const int src = g->cur, dst = 1 - g->cur;
     BCArgs a{g->bc, g->nbc, g->u_in, g->rho_in, g->delU, ramp};
@@ -416,12 +453,17 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         s.omega = g->cs.omega;
         s.cs2 = g->cs.cs2;
         s.cs4 = g->cs.cs4;
-        const double rho_init = 1.0;
-        const double DxDt = g->cs.Dx * g->cs.Dt;
-        s.Fx = (rho_init * g->cs.Drho * p->gravityX + p->dpdx) * (DxDt * DxDt) / g->cs.Dm;
-        s.Fy = (rho_init * g->cs.Drho * p->gravityY + p->dpdy) * (DxDt * DxDt) / g->cs.Dm;
+        if (p->womersley != 0.0 && (p->gravityX != 0.0 || p->gravityY != 0.0))
+            throw std::runtime_error("WOMERSLEY forcing with gravity is not supported (force_xy would vary "
+                                     "from site to site)");
+        double F[2];
+        body_force(*g, 0, F);
+        s.Fx = F[0];
+        s.Fy = F[1];
         s.Fx0 = s.Fx + 0.0;
         s.Fy0 = s.Fy + 0.0;
+        s.Fxm = s.Fx;
+        s.Fym = s.Fy;
         s.Dx = g->cs.Dx;
         s.velScale = g->cs.Dt / g->cs.Dx;
 

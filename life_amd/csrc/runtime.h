@@ -100,6 +100,7 @@ struct life_grid {
     hipEvent_t ev_ready = nullptr, ev_bulk = nullptr, ev_tail = nullptr;
     bool bulk_pending = false;  // a bulk kernel of the next step is queued on stream2
     bool tail_pending = false;  // the last step's tail ran on stream2 (stream not yet joined)
+    int t_step = 0;             // time step of the last lbm_step
 
     void* f_cur() { return f[cur]; }
 };
@@ -123,6 +124,12 @@ void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end);
 // then makes the grid's stream wait for that work before it reads the
 // lattice (interpolation, downloads, the end of a call).
 void lbm_join(life_grid* g);
+// force_xy of time step t (uniform): the initial value (Grid.cpp:1042-1043),
+// or under WOMERSLEY the oscillating pressure gradient of step t
+// (Grid.cpp:52-61).
+void body_force(const life_grid& g, int t, double F[2]);
+// The grid's kernel scalars for time step t: force_xy of t and of t - 1.
+GridScalars step_scalars(const life_grid& g, int t);
 void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);

@@ -295,6 +295,7 @@ void fill_table(life_server* s) {
         const Slot& sl = s->slots[q];
         life_grid* g = sl.c->g;
         CaseDev& d = s->cases.h[q];
+        g->g = step_scalars(*g, sl.c->t + 1);  // this step's force_xy (WOMERSLEY) for later host-side use
         d.g = g->g;
         d.src = g->f[g->cur];
         d.dst = g->f[1 - g->cur];

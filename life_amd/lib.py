@@ -32,6 +32,7 @@ class LifeParams(C.Structure):
         ("initial_deflect", C.c_double),
         ("ref_nu", C.c_double), ("ref_rho", C.c_double), ("ref_P", C.c_double),
         ("ref_L", C.c_double), ("ref_U", C.c_double),
+        ("womersley", C.c_double),
     ]
 
 

@@ -607,6 +607,17 @@ int orc_lbm_kernel(orc_case *c) {
     swapp(&c->f_n, &c->f);
     swapp(&c->u_n, &c->u);
     swapp(&c->rho_n, &c->rho);
+    /* WOMERSLEY: oscillating pressure-gradient forcing, Grid.cpp:52-61 */
+    if (c->p.womersley != 0.0) {
+        const orc_params *p = &c->p;
+        const double period = ((p->height_p * p->height_p) * M_PI) / (2.0 * (p->womersley * p->womersley) * p->nu_p);
+        const double wave = cos(2.0 * M_PI * c->t * c->Dt / period);
+        const double sc = (c->Dx * c->Dt) * (c->Dx * c->Dt);
+        for (long id = 0; id < N; id++) {
+            c->fxy[id * 2] = (c->rho_n[id] * c->Drho * p->gravityX + p->dpdx * wave) * sc / c->Dm;
+            c->fxy[id * 2 + 1] = (c->rho_n[id] * c->Drho * p->gravityY + p->dpdy * wave) * sc / c->Dm;
+        }
+    }
     const double omega = c->omega, cs2 = c->cs2, cs4 = c->cs4;
     double *f = c->f;
     const double *fn = c->f_n, *un = c->u_n, *rn = c->rho_n, *fxy = c->fxy, *fib = c->fibm;

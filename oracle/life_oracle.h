@@ -37,6 +37,7 @@ typedef struct orc_params {
     int uni_epsilon;                /* UNI_EPSILON */
     double initial_deflect;         /* INITIAL_DEFLECT, 0 = undefined */
     double ref_nu, ref_rho, ref_P, ref_L, ref_U;
+    double womersley;               /* WOMERSLEY, 0 = undefined */
 } orc_params;
 
 typedef struct orc_case orc_case;

@@ -4,7 +4,9 @@ lid-driven cavity 256x256 (BGK), rigid cylinder in a 1024x256 channel
 (central moments, parabolic inlet, convective outlet), and the cylinder +
 flexible plate FSI case on 2048x410 (uniform epsilon system, FP64 FEM,
 Aitken coupling), with the inlet at full speed from the first step so that
-the coupling subiterates.  Fields, forces and structure at 1e-12 relative."""
+the coupling subiterates; and the reference's PELskin example at its own
+size (735x91, physical units, ten flexible filaments in the oscillating
+Womersley channel).  Fields, forces and structure at 1e-12 relative."""
 import pytest
 
 from life_amd import config as cfg
@@ -21,6 +23,7 @@ CASES = {
     "ldc256": (lambda: cfg.lid_driven_cavity(), 200),
     "cylinder1024": (lambda: cfg.cylinder_channel(ramp_steps=0.0), 60),
     "fsi2048": (lambda: cfg.flexible_fsi(ramp_steps=0.0), 15),
+    "pelskin735": (lambda: cfg.pelskin(), 80),
 }
 
 

@@ -9,6 +9,7 @@ oracle through the velocity L2 error.
 import numpy as np
 import pytest
 
+from life_amd import config as cfg
 from tests._util import BC_KINDS, bc_mix_params, random_state, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -77,3 +78,48 @@ def test_fp32_shifted_accuracy(oracle, gpu_lib, kind, cm):
     print("u L2 rel", l2, "rho max", rel_err(rg, ro))
     assert l2 <= 1e-5
     assert rel_err(rg, ro) <= 1e-6
+
+
+def womersley_params(kind, cm):
+    if kind == "periodic":
+        return cfg.womersley_channel(Nx=24, Ny=33, central_moments=cm), ""
+    # inlet, convective outlet and an oscillating force together
+    p, g = bc_mix_params("channel_convective", cm=cm)
+    return p.replace(dpdx=2e-5, womersley=3.0), g
+
+
+@pytest.mark.parametrize("cm", [0, 1])
+@pytest.mark.parametrize("kind", ["periodic", "convective"])
+def test_womersley_matches_oracle(oracle, gpu_lib, kind, cm):
+    """WOMERSLEY forcing (Grid.cpp:52-61): the force of step t enters the
+    collision and that of step t - 1 the moments of the incoming
+    populations; FP64 equals the oracle at 1e-12."""
+    p, g = womersley_params(kind, cm)
+    oc, gc = run_both(oracle, p, g, 80)
+    for _ in range(2):
+        fo, ro, uo, _ = oc.grid()
+        fg, rg, ug, _ = gc.fields()
+        assert rel_err(rg, ro) <= TOL
+        assert rel_err(ug, uo) <= TOL
+        assert rel_err(fg, fo) <= TOL
+        oc.step(70)
+        gc.step(70)
+
+
+def test_womersley_fp32_shifted_accuracy(oracle, gpu_lib):
+    p, g = womersley_params("convective", 1)
+    oc, gc = run_both(oracle, p, g, 200, precision=1)
+    _, ro, uo, _ = oc.grid()
+    _, rg, ug, _ = gc.fields()
+    l2 = np.linalg.norm(ug - uo) / np.linalg.norm(uo)
+    print("u L2 rel", l2, "rho max", rel_err(rg, ro))
+    assert l2 <= 1e-5
+    assert rel_err(rg, ro) <= 1e-6
+
+
+def test_womersley_rejects_gravity(gpu_lib):
+    from life_amd.case import Case
+    from life_amd.lib import LifeError
+    p = cfg.womersley_channel().replace(gravityX=1e-6)
+    with pytest.raises(LifeError, match="WOMERSLEY"):
+        Case(p, "", precision=0)

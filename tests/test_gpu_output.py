@@ -39,11 +39,17 @@ def state(c):
     return out
 
 
-@pytest.mark.parametrize("which", ["fsi", "canopy"])
+def womersley():
+    p, g = cfg.womersley_cylinder()
+    return p.replace(womersley=8.0), g
+
+
+@pytest.mark.parametrize("which", ["fsi", "canopy", "womersley"])
 def test_restart_continues_bitwise(tmp_path, which):
     """Run A for n1 + n2 steps; run B for n1 steps, write the restart, and
-    run C from that restart for n2 steps: C equals A bit for bit."""
-    p, g = fsi() if which == "fsi" else canopy()
+    run C from that restart for n2 steps: C equals A bit for bit (for the
+    oscillating force too: it resumes at the restart's time step)."""
+    p, g = {"fsi": fsi, "canopy": canopy, "womersley": womersley}[which]()
     n1, n2 = 9, 7
     a = Case(p, g, precision=L.LIFE_FP64)
     a.step(n1 + n2)

@@ -103,6 +103,25 @@ def test_alone_equals_mix(graph):
     mix.close()
 
 
+@pytest.mark.parametrize("prec", [L.LIFE_FP64, L.LIFE_FP32_SHIFTED])
+def test_womersley_served_matches_case(prec):
+    """An oscillating-force case (WOMERSLEY) admitted after the engine has
+    already stepped: the force follows the case's own time step, not the
+    engine's, so it equals its own life_case run."""
+    pw, gw = cfg.womersley_cylinder()
+    pw = pw.replace(womersley=8.0)
+    steps = 15
+    ref = run_case((pw, gw), steps, prec)
+    srv = Server(precision=prec, max_cases=4)
+    srv.submit(*cylinder(), 30)
+    for _ in range(4):
+        srv.step()
+    rid = srv.submit(pw, gw, steps, keep_fields=True)
+    srv.drain()
+    assert_same(served(srv, rid), ref)
+    srv.close()
+
+
 def test_admission_by_budget():
     """With room for one case at a time the others wait in arrival order;
     every request still finishes with the single-case result."""

@@ -173,6 +173,35 @@ def test_poiseuille(oracle, cm):
     assert np.max(np.abs(u[:, :, 1])) < 1e-12
 
 
+@pytest.mark.parametrize("cm", [0, 1])
+def test_womersley_channel(oracle, cm):
+    """WOMERSLEY forcing (Grid.cpp:52-61): after the start-up transient has
+    decayed (rate nu pi^2 / H^2), the channel velocity follows the planar
+    Womersley solution through a whole period; a wrong period, sign or
+    amplitude of the oscillating force fails it."""
+    Ny, tau, Wo, G = 33, 0.8, 4.0, 1e-5
+    p = cfg.womersley_channel(Ny=Ny, tau=tau, Wo=Wo, G=G, central_moments=cm)
+    nu = cfg.lattice_viscosity(1.0 / tau)
+    H = Ny - 1.0
+    period = H * H * np.pi / (2.0 * Wo * Wo * nu)
+    c = oracle.OracleCase(p, "")
+    t = 9000
+    c.step(t)
+    y = np.arange(Ny, dtype=float)
+    amp = G / (2.0 * np.pi / period)  # the inviscid amplitude G / omega
+    worst = 0.0
+    for _ in range(8):
+        c.step(int(period / 8))
+        t += int(period / 8)
+        _, _, u, _ = c.grid()
+        exact = np.array(cfg.womersley_profile(y, float(t), H, nu, Wo, G))
+        worst = max(worst, np.max(np.abs(u[1, :, 0] - exact)) / amp)
+        assert np.max(np.abs(u[:, :, 1])) < 1e-6 * amp
+    print("womersley worst", worst)
+    assert worst < 1e-2
+    c.close()
+
+
 def test_regularised_wall_velocity(oracle):
     """After a step, the rebuilt wall populations carry the imposed velocity
     exactly (regularised BC, Latt et al. 2008)."""
