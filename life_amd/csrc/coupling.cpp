@@ -489,10 +489,13 @@ int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_
         LIFE_HIP(hipSetDevice(g->device));
         LIFE_HIP(hipStreamSynchronize(g->stream));
         // one event pair per step around the stream-collide launch, read
-        // after the last step so the host never waits inside the loop
+        // after the last step so the host never waits inside the loop.
+        // They order nothing, so they are recorded without the system-scope
+        // fence (cache writeback and invalidate) that a default event adds
+        // and that the untimed step does not have
         while ((int)c->kev.size() < 2 * nsteps) {
             hipEvent_t e;
-            LIFE_HIP(hipEventCreate(&e));
+            LIFE_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             c->kev.push_back(e);
         }
         LIFE_HIP(hipEventRecord(g->ev_a, g->stream));
