@@ -205,17 +205,19 @@ def bench_serve(args):
           "plate_length": "0.25*height", "fem_elements": [20, 60], "Re": [50, 400], "K_B": [0.1, 1.0],
           "collision": "central_moments", "seed": 20261203, "slots": args.serve_slots,
           "hbm_budget_bytes": args.serve_budget, "precision": "fp32_shifted populations, fp64 FEM/coupling"}
-    srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
-    print("bench: serving %d cases x %d steps" % (args.serve_cases, args.serve_steps), file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    ids = srv.run(stream)
-    wall = time.perf_counter() - t0
-    full, _ = serve_summary(srv, ids, wall)
-    srv.close()
-    print("bench: serving done in %.1f s: %s" % (wall, json.dumps(full)), file=sys.stderr, flush=True)
+    full = None
+    if not args.capacity_only:
+        srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
+        print("bench: serving %d cases x %d steps" % (args.serve_cases, args.serve_steps), file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ids = srv.run(stream)
+        wall = time.perf_counter() - t0
+        full, _ = serve_summary(srv, ids, wall)
+        srv.close()
+        print("bench: serving done in %.1f s: %s" % (wall, json.dumps(full)), file=sys.stderr, flush=True)
 
     base = None
-    if args.baseline_cases > 0:
+    if args.baseline_cases > 0 and not args.capacity_only:
         pre = stream[:args.baseline_cases]
         srv = Server(precision=lib.LIFE_FP32_SHIFTED, max_cases=args.serve_slots, hbm_budget=args.serve_budget)
         t0 = time.perf_counter()
@@ -233,7 +235,7 @@ def bench_serve(args):
                 "p99_ratio": plain["p99_s"] / served["p99_s"],
                 "accuracy": serve_accuracy(pre, pinfo, pout, args)}
     capacity = None
-    if args.serve_capacity:
+    if args.serve_capacity or args.capacity_only:
         # the same cases all queued at once: the engine's own throughput,
         # not bounded by the arrival rate
         burst = [dict(r, arrival=0.0) for r in stream]
@@ -246,6 +248,15 @@ def bench_serve(args):
         if base is not None:
             capacity["ratio_vs_plain_fp64"] = capacity["cases_per_hour"] / base["plain_fp64_one_at_a_time"]["cases_per_hour"]
         print("bench: burst served in %.1f s" % bwall, file=sys.stderr, flush=True)
+    if full is None:
+        # capacity only: the same cases all queued at once
+        out = {"metric": "FSI cases/hour (serving engine, all cases queued at once)",
+               "value": capacity["cases_per_hour"], "unit": "cases/hour", "n_gpus": 1, "steps": args.serve_steps,
+               "warmup": 0, "ms_per_step": capacity["ms_per_engine_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": wl,
+               "capacity_all_queued_at_once": capacity}
+        print(json.dumps(out))
+        return
     out = {
         "metric": "FSI cases/hour (serving engine, continuous batching)",
         "value": full["cases_per_hour"],
@@ -349,6 +360,8 @@ def main():
     ap.add_argument("--baseline-cases", type=int, default=32)
     ap.add_argument("--accuracy-cases", type=int, default=8)
     ap.add_argument("--accuracy-steps", type=int, default=300)
+    ap.add_argument("--capacity-only", action="store_true",
+                    help="serving: only the run with every case queued at once")
     ap.add_argument("--serve-capacity", action="store_true",
                     help="also run the same cases all queued at once (engine throughput without arrivals)")
     args = ap.parse_args()

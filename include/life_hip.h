@@ -241,8 +241,9 @@ const char *life_server_request_error(const life_server *s, int id);
 int life_server_fields(const life_server *s, int id, double *rho, double *u, double *marker_pos);
 int life_server_release(life_server *s, int id);
 /* site updates, engine steps, busy seconds, graph captures, peak running
- * cases, peak admitted bytes */
-int life_server_stats(const life_server *s, double out[6]);
+ * cases, peak admitted bytes, engine steps whose next bulk stream-collide
+ * was launched early (beside the coupling) */
+int life_server_stats(const life_server *s, double out[7]);
 
 #ifdef __cplusplus
 }

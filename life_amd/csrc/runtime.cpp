@@ -335,6 +335,41 @@ static void free_markers(life_grid* g) {
     g->flags = nullptr;
 }
 
+// The stream is masked to all CUs but the reserve.  The reserved CUs are
+// spread over the 8 XCDs whether the mask bits number the CUs XCD by XCD or
+// round-robin over the XCDs: CU x*G + ((x + 8*k) mod G) for XCD x and
+// k = 0, 1, ... (G = CUs per XCD) falls on XCD x either way.  A CU-masked
+// stream has a hardware queue of its own, so there is one per device and
+// reserve for the whole process, shared by every grid or server on it
+// (never destroyed).  Sharing cannot deadlock: a user records the event
+// its kernel waits for before it queues that wait.
+hipStream_t masked_stream(int device, int reserve_cus) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, hipStream_t> streams;
+    std::lock_guard<std::mutex> lock(mu);
+    int ncu = 0;
+    LIFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    const int reserve = reserve_cus < 0 ? 0 : (reserve_cus < ncu / 8 ? reserve_cus : ncu / 8);  // at most G/8 per XCD
+    auto it = streams.find({device, reserve});
+    if (it != streams.end()) return it->second;
+    hipStream_t st = nullptr;
+    if (reserve == 0 || ncu % 8 != 0) {
+        LIFE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    } else {
+        const int G = ncu / 8;
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+        for (int c = 0; c < ncu; c++) mask[(size_t)c / 32] |= 1u << (c % 32);
+        for (int r = 0; r < reserve; r++) {
+            const int x = r % 8, k = r / 8;
+            const int c = x * G + (x + 8 * k) % G;
+            mask[(size_t)c / 32] &= ~(1u << (c % 32));
+        }
+        LIFE_HIP(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
+    }
+    streams[{device, reserve}] = st;
+    return st;
+}
+
 }  // namespace life
 
 using namespace life;
@@ -369,14 +404,6 @@ int coupling_cus() {
     return n;
 }
 
-// The second stream, masked to all CUs but the reserve.  The reserved CUs
-// are spread over the 8 XCDs whether the mask bits number the CUs XCD by
-// XCD or round-robin over the XCDs: CU x*G + ((x + 8*k) mod G) for XCD x
-// and k = 0, 1, ... (G = CUs per XCD) falls on XCD x either way.
-// A CU-masked stream has a hardware queue of its own, so there is one per
-// device for the whole process, shared by every grid on it (never
-// destroyed).  Sharing cannot deadlock: a grid records the event its bulk
-// kernel waits for before it queues that wait.
 // Wave priority of the coupling kernels (LIFE_COUPLING_PRIO, 0..3, default
 // 3), set once per device.
 void init_coupling_priority(int device) {
@@ -391,33 +418,10 @@ void init_coupling_priority(int device) {
     done[device] = true;
 }
 
-hipStream_t bulk_stream(int device) {
-    static std::mutex mu;
-    static std::map<int, hipStream_t> streams;
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = streams.find(device);
-    if (it != streams.end()) return it->second;
-    int ncu = 0;
-    LIFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-    const int reserve = coupling_cus() < ncu / 8 ? coupling_cus() : ncu / 8;  // at most G/8 per XCD
-    hipStream_t st = nullptr;
-    if (reserve == 0 || ncu % 8 != 0) {
-        LIFE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    } else {
-        const int G = ncu / 8;
-        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
-        for (int c = 0; c < ncu; c++) mask[(size_t)c / 32] |= 1u << (c % 32);
-        for (int r = 0; r < reserve; r++) {
-            const int x = r % 8, k = r / 8;
-            const int c = x * G + (x + 8 * k) % G;
-            mask[(size_t)c / 32] &= ~(1u << (c % 32));
-        }
-        LIFE_HIP(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
-    }
-    streams[device] = st;
-    return st;
-}
+// The second stream of a grid: the early bulk stream-collide.
+hipStream_t bulk_stream(int device) { return masked_stream(device, coupling_cus()); }
 }  // namespace
+
 
 int life_grid_create(const life_params* p, int precision, int device, life_grid** out) {
     *out = nullptr;

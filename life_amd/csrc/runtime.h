@@ -124,6 +124,11 @@ void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end);
 // then makes the grid's stream wait for that work before it reads the
 // lattice (interpolation, downloads, the end of a call).
 void lbm_join(life_grid* g);
+// A second stream on `device` whose kernels leave reserve_cus CUs (at most
+// one eighth of them, spread over the XCDs) to the other streams' work.
+// One per (device, reserve) for the whole process, never destroyed: a
+// CU-masked stream holds a hardware queue of its own.
+hipStream_t masked_stream(int device, int reserve_cus);
 // force_xy of time step t (uniform): the initial value (Grid.cpp:1042-1043),
 // or under WOMERSLEY the oscillating pressure gradient of step t
 // (Grid.cpp:52-61).

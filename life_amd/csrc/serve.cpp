@@ -6,9 +6,14 @@
 // footprint fits the HBM budget.  Every engine step advances each admitted
 // case by one timestep:
 //  - one table of per-case descriptors (CaseDev) goes to the device;
-//  - the lattice phase of all cases (outlet speed, stream-collide over a
-//    packed (case, block) index, support-site re-push, boundaries) is one
-//    hipGraph launch, captured again only when the set of cases changes;
+//  - the lattice phase of all cases is two hipGraph launches, captured
+//    again only when the set of cases changes: the bulk stream-collide over
+//    a packed (case, block) index, then the tail (outlet speed, support-site
+//    re-push, boundaries);
+//  - when the next step will advance the same cases (none finishes, none
+//    can be admitted), the next step's bulk stream-collide is launched on a
+//    second stream as soon as this step's lattice phase is done, so it runs
+//    beside this step's coupling; the next step then runs only the tail;
 //  - the coupling loop (ObjectsClass::objectKernel, Objects.cpp:26-60) runs
 //    in lockstep: the device work of every case still subiterating is one
 //    batched launch per kernel, marker data crosses the bus in one packed
@@ -22,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -175,8 +181,20 @@ struct life_server {
     LatticePlan plan;
     int off_conv = 0, off_ibm[2] = {0, 0}, off_bc[2] = {0, 0}, off_ibm_all = 0, n_ibm_all = 0;
     int total_markers = 0, max_markers = 0, max_bodies = 0;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;  // lattice tail
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_bulk = nullptr;  // bulk stream-collide
+    hipGraphExec_t exec_bulk = nullptr;
+    // early bulk stream-collide of the next step (environment
+    // LIFE_SERVE_EARLY): 0 off, 1 launched when the lattice phase is
+    // queued, 2 launched when the host FEM of the first subiteration starts
+    // (the device is otherwise idle then)
+    int early = 2;
+    bool prepushed = false;         // this step's bulk stream-collide already ran on stream2
+    hipStream_t stream2 = nullptr;  // shared per device (masked_stream), not owned
+    hipEvent_t ev_lattice = nullptr, ev_bulk = nullptr;
+    Mirror<CaseDev> cases_bulk;     // descriptors the bulk stream-collide reads
+    double early_steps = 0.0;
     EngineTimes times;
     // statistics
     double site_updates = 0.0, steps = 0.0, busy = 0.0, captures = 0.0, peak_running = 0.0, peak_bytes = 0.0;
@@ -189,8 +207,12 @@ namespace {
 void drop_graph(life_server* s) {
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->exec_bulk) (void)hipGraphExecDestroy(s->exec_bulk);
+    if (s->graph_bulk) (void)hipGraphDestroy(s->graph_bulk);
     s->exec = nullptr;
     s->graph = nullptr;
+    s->exec_bulk = nullptr;
+    s->graph_bulk = nullptr;
 }
 
 void fail(life_server* s, Slot& sl, const std::string& msg) {
@@ -199,12 +221,18 @@ void fail(life_server* s, Slot& sl, const std::string& msg) {
     (void)s;
 }
 
+// The head of the queue fits a free slot and the budget.
+bool can_admit(const life_server* s) {
+    if (s->queue.empty() || (int)s->slots.size() >= s->cfg.max_cases) return false;
+    const double bytes = case_bytes(s->queue.front()->p, s->cfg.precision);
+    return s->slots.empty() || s->used_bytes + bytes <= s->cfg.hbm_budget;
+}
+
 // Admit queued requests in arrival order while a slot and budget remain.
 void admit(life_server* s) {
-    while (!s->queue.empty() && (int)s->slots.size() < s->cfg.max_cases) {
+    while (can_admit(s)) {
         Request* r = s->queue.front();
         const double bytes = case_bytes(r->p, s->cfg.precision);
-        if (!s->slots.empty() && s->used_bytes + bytes > s->cfg.hbm_budget) break;
         s->queue.pop_front();
         r->info.t_admit = s->now();
         r->info.bytes = bytes;
@@ -281,6 +309,7 @@ void rebuild(life_server* s) {
     std::copy(all.begin(), all.end(), s->lists.h);
     s->lists.upload(all.size(), s->stream);
     s->cases.reserve(s->slots.size());
+    s->cases_bulk.reserve(s->slots.size());
     s->stage_in.reserve(4 * (size_t)s->total_markers);
     s->stage_out.reserve(3 * (size_t)s->total_markers);
     s->dyn.reserve(s->slots.size());
@@ -317,21 +346,29 @@ void fill_table(life_server* s) {
 
 // GridClass::lbmKernel of every case (Grid.cpp:36-100), same stage order
 // as lbm_step: outlet speed, stream-collide, support sites, boundaries.
-void launch_lattice(life_server* s) {
+// Bulk stream-collide of every case (descriptors: cases_bulk).
+void launch_bulk(life_server* s, hipStream_t st) {
     const LatticePlan& P = s->plan;
     const bool f64 = s->cfg.precision == LIFE_FP64;
-    hipStream_t st = s->stream;
+    for (int k = 0; k < 2; k++) {
+        const int nb = (int)P.blocks[k].size();
+        if (f64) launch_push64_batch(k == 1, s->cases_bulk.d, s->blocks[k].d, nb, st);
+        else launch_push32_batch(k == 1, s->cases_bulk.d, s->blocks[k].d, nb, st);
+    }
+    LIFE_HIP(hipGetLastError());
+}
+
+// Lattice tail: outlet speed (reads the incoming populations only, so it
+// may follow the bulk kernel), support-site re-push, boundaries.
+void launch_tail(life_server* s, hipStream_t st) {
+    const LatticePlan& P = s->plan;
+    const bool f64 = s->cfg.precision == LIFE_FP64;
     const int* L = s->lists.d;
     if (!P.conv.empty()) {
         if (f64)
             launch_convective_batch<double>(s->cases.d, L + s->off_conv, (int)P.conv.size(), P.max_ny, true, st);
         else
             launch_convective_batch<float>(s->cases.d, L + s->off_conv, (int)P.conv.size(), P.max_ny, false, st);
-    }
-    for (int k = 0; k < 2; k++) {
-        const int nb = (int)P.blocks[k].size();
-        if (f64) launch_push64_batch(k == 1, s->cases.d, s->blocks[k].d, nb, st);
-        else launch_push32_batch(k == 1, s->cases.d, s->blocks[k].d, nb, st);
     }
     for (int k = 0; k < 2; k++) {
         const int n = (int)P.ibm[k].size();
@@ -346,26 +383,72 @@ void launch_lattice(life_server* s) {
     LIFE_HIP(hipGetLastError());
 }
 
-void lattice_phase(life_server* s) {
+// Run body(stream) on `on`: directly, or as a graph captured on the
+// engine stream the first time and launched on `on`.
+template <typename Body>
+void run_phase(life_server* s, hipGraph_t& graph, hipGraphExec_t& exec, hipStream_t on, Body&& body) {
     if (!s->cfg.use_graph) {
-        launch_lattice(s);
+        body(on);
         return;
     }
-    if (!s->exec) {
+    if (!exec) {
         LIFE_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         try {
-            launch_lattice(s);
+            body(s->stream);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(s->stream, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        LIFE_HIP(hipStreamEndCapture(s->stream, &s->graph));
-        LIFE_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+        LIFE_HIP(hipStreamEndCapture(s->stream, &graph));
+        LIFE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         s->captures += 1.0;
     }
-    LIFE_HIP(hipGraphLaunch(s->exec, s->stream));
+    LIFE_HIP(hipGraphLaunch(exec, on));
+}
+
+void lattice_phase(life_server* s) {
+    if (s->prepushed) {
+        // the bulk stream-collide of this step ran on stream2 beside the
+        // previous step's coupling
+        LIFE_HIP(hipStreamWaitEvent(s->stream, s->ev_bulk, 0));
+        s->prepushed = false;
+    } else {
+        std::copy(s->cases.h, s->cases.h + s->slots.size(), s->cases_bulk.h);
+        s->cases_bulk.upload(s->slots.size(), s->stream);
+        run_phase(s, s->graph_bulk, s->exec_bulk, s->stream, [&](hipStream_t st) { launch_bulk(s, st); });
+    }
+    run_phase(s, s->graph, s->exec, s->stream, [&](hipStream_t st) { launch_tail(s, st); });
+}
+
+// Next step's bulk stream-collide on stream2, beside this step's coupling,
+// when the next step advances exactly the cases of this one: none finishes
+// now and none can be admitted.  It reads the populations this step's
+// lattice phase wrote and writes the other buffer, which the coupling does
+// not touch; the support sites and boundaries are redone by the next
+// step's tail after the spread, as in the plain order.
+void launch_bulk_early(life_server* s) {
+    if (s->prepushed || s->slots.empty() || can_admit(s)) return;
+    for (const Slot& sl : s->slots)
+        if (sl.c->t >= sl.req->nsteps || sl.req->info.status == LIFE_REQ_FAILED) return;
+    for (int q = 0; q < (int)s->slots.size(); q++) {
+        const life_case* c = s->slots[q].c;
+        const life_grid* g = c->g;
+        CaseDev& d = s->cases_bulk.h[q];
+        d = s->cases.h[q];
+        d.g = step_scalars(*g, c->t + 1);
+        d.src = g->f[g->cur];
+        d.dst = g->f[1 - g->cur];
+        d.explicit_state = 0;
+    }
+    LIFE_HIP(hipEventRecord(s->ev_lattice, s->stream));
+    LIFE_HIP(hipStreamWaitEvent(s->stream2, s->ev_lattice, 0));
+    s->cases_bulk.upload(s->slots.size(), s->stream2);
+    run_phase(s, s->graph_bulk, s->exec_bulk, s->stream2, [&](hipStream_t st) { launch_bulk(s, st); });
+    LIFE_HIP(hipEventRecord(s->ev_bulk, s->stream2));
+    s->prepushed = true;
+    s->early_steps += 1.0;
 }
 
 // Upload the host marker positions/velocities of the listed cases.
@@ -518,7 +601,9 @@ void coupling_phase(life_server* s) {
             case_solve_eps_dense(c, dense[k].second, dense_n[k]);
             LIFE_HIP(hipStreamSynchronize(c->g->stream));
         }
-        // femKernel on host threads, then the convergence test per case
+        // femKernel on host threads, then the convergence test per case; the
+        // device is idle meanwhile, so the next step's bulk kernel starts now
+        if (s->early == 2) launch_bulk_early(s);
         s->times.lap(st, 3);
         s->times.subits += 1;
         for_cases(s, active, [&](Slot& sl) { case_fem(sl.c); });
@@ -580,7 +665,12 @@ void retire(life_server* s, Slot& sl) {
 
 void engine_step(life_server* s) {
     s->times.start(s->stream);
-    admit(s);
+    // after an early bulk launch the set of cases stays as it was: only a
+    // failure in the last coupling can have freed a slot, and a case
+    // admitted now would miss its bulk stream-collide, so admission waits
+    // one step
+    if (s->prepushed && s->slots.empty()) s->prepushed = false;
+    if (!s->prepushed) admit(s);
     if (s->slots.empty()) return;
     const double t_begin = s->now();
     if (s->dirty) rebuild(s);
@@ -595,8 +685,15 @@ void engine_step(life_server* s) {
         c->g->explicit_state = false;
         s->site_updates += (double)c->g->g.N;
     }
+    if (s->early == 1) launch_bulk_early(s);
     coupling_phase(s);
+    // no FEM this step (or it failed): the next bulk kernel still runs
+    // beside the spread and the step's end
+    if (s->early == 2) launch_bulk_early(s);
     LIFE_HIP(hipStreamSynchronize(s->stream));
+    // a case that retires below may still be read by the early kernel, and
+    // cases_bulk's host copy is written again next step
+    LIFE_HIP(hipStreamSynchronize(s->stream2));
     s->times.lap(s->stream, 5);
     s->times.steps += 1;
     s->steps += 1.0;
@@ -657,6 +754,14 @@ int life_server_create(const life_server_config* cfg, life_server** out) {
         s->cfg = *cfg;
         LIFE_HIP(hipSetDevice(cfg->device));
         LIFE_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        // CUs the early bulk kernel leaves to the coupling kernels beside it
+        // (environment LIFE_SERVE_COUPLING_CUS, default 16 of 256: measured
+        // best on MI355X among 0, 16 and 32)
+        const char* rc_env = std::getenv("LIFE_SERVE_COUPLING_CUS");
+        s->stream2 = masked_stream(cfg->device, rc_env ? std::atoi(rc_env) : 16);
+        LIFE_HIP(hipEventCreateWithFlags(&s->ev_lattice, hipEventDisableTiming));
+        LIFE_HIP(hipEventCreateWithFlags(&s->ev_bulk, hipEventDisableTiming));
+        if (const char* e = std::getenv("LIFE_SERVE_EARLY")) s->early = std::atoi(e);
         s->t0 = Clock::now();
     });
     if (rc) {
@@ -672,11 +777,13 @@ int life_server_destroy(life_server* s) {
     const int rc = guarded([&] {
         (void)hipSetDevice(s->cfg.device);
         if (s->stream) (void)hipStreamSynchronize(s->stream);
+        if (s->stream2) (void)hipStreamSynchronize(s->stream2);
         s->times.report();
         for (Slot& sl : s->slots) life_case_destroy(sl.c);
         s->slots.clear();
         drop_graph(s);
         s->cases.release();
+        s->cases_bulk.release();
         for (auto& b : s->blocks) b.release();
         s->lists.release();
         s->dyn.release();
@@ -684,6 +791,8 @@ int life_server_destroy(life_server* s) {
         s->outside.release();
         s->stage_in.release();
         s->stage_out.release();
+        if (s->ev_lattice) (void)hipEventDestroy(s->ev_lattice);
+        if (s->ev_bulk) (void)hipEventDestroy(s->ev_bulk);
         if (s->stream) (void)hipStreamDestroy(s->stream);
     });
     delete s;
@@ -758,13 +867,14 @@ int life_server_release(life_server* s, int id) {
     });
 }
 
-int life_server_stats(const life_server* s, double out[6]) {
+int life_server_stats(const life_server* s, double out[7]) {
     out[0] = s->site_updates;
     out[1] = s->steps;
     out[2] = s->busy;
     out[3] = s->captures;
     out[4] = s->peak_running;
     out[5] = s->peak_bytes;
+    out[6] = s->early_steps;
     return 0;
 }
 

@@ -87,7 +87,7 @@ class Server:
         L.check(L.load().life_server_release(self.h, int(rid)))
 
     def stats(self):
-        out = np.zeros(6)
+        out = np.zeros(7)
         L.check(L.load().life_server_stats(self.h, L.dptr(out)))
-        keys = ["site_updates", "steps", "busy_s", "graph_captures", "peak_running", "peak_bytes"]
+        keys = ["site_updates", "steps", "busy_s", "graph_captures", "peak_running", "peak_bytes", "early_steps"]
         return dict(zip(keys, out.tolist()))

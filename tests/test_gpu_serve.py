@@ -122,6 +122,32 @@ def test_womersley_served_matches_case(prec):
     srv.close()
 
 
+@pytest.mark.parametrize("prec,graph", [(L.LIFE_FP64, True), (L.LIFE_FP32_SHIFTED, True),
+                                        (L.LIFE_FP32_SHIFTED, False)])
+def test_early_bulk_matches_plain(monkeypatch, prec, graph):
+    """The next step's bulk stream-collide launched beside the coupling
+    (while the set of cases stays fixed), as soon as the lattice phase is
+    queued (mode 1) or when the host FEM starts (mode 2, the default), gives
+    the same results, bit for bit, as the plain order, through admissions
+    and retirements."""
+    pw, gw = cfg.womersley_cylinder()
+    work = [(flag(), 10), (cylinder(), 14), (cavity(), 7), (canopy(), 9), ((pw.replace(womersley=8.0), gw), 12)]
+    out = {}
+    for early in ("0", "1", "2"):
+        monkeypatch.setenv("LIFE_SERVE_EARLY", early)
+        srv = Server(precision=prec, max_cases=3, use_graph=graph)
+        ids = [srv.submit(p, g, n, keep_fields=True) for (p, g), n in work]
+        srv.drain()
+        out[early] = [served(srv, rid) for rid in ids]
+        st = srv.stats()
+        print("early", early, "early steps", st["early_steps"], "of", st["steps"], flush=True)
+        assert (st["early_steps"] > 0) == (early != "0")
+        srv.close()
+    for mode in ("1", "2"):
+        for a, b in zip(out["0"], out[mode]):
+            assert_same(a, b)
+
+
 def test_admission_by_budget():
     """With room for one case at a time the others wait in arrival order;
     every request still finishes with the single-case result."""
