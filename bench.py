@@ -281,11 +281,43 @@ def bench_serve(args):
 CONFIGS = {
     # name: (builder, default steps, oracle parity steps, description)
     "ldc256": ("lid_driven_cavity", 20000, 200, "lid-driven cavity 256x256, BGK, tau 0.6, lid 0.05"),
-    "cylinder1024": ("cylinder_channel", 20000, 100,
+    "cylinder1024": ("cylinder_channel", 50000, 100,
                      "rigid cylinder D=40 in a 1024x256 channel, central moments, Re 100, parabolic inlet"),
     "fsi2048": ("flexible_fsi", 4000, 30,
                 "cylinder D=80 + flexible plate L=280 h=16 (40 elements) in 2048x410, central moments, Re 100"),
 }
+
+
+def coefficient_summary(series, p):
+    """Force coefficients over the second half of a run: mean drag, lift
+    amplitude and rms, and the shedding Strouhal number from the lift's
+    upward zero crossings (St = f D / U with the case's reference length and
+    velocity, time in units of Dt)."""
+    import numpy as np
+    t, cd, cl = series[:, 0], series[:, 3], series[:, 4]
+    h = len(t) // 2
+    t, cd, cl = t[h:], cd[h:], cl[h:]
+    out = {"samples": int(len(t)), "from_step": int(t[0]), "to_step": int(t[-1]), "cd_mean": float(cd.mean()),
+           "cl_mean": float(cl.mean()), "cl_rms": float(np.sqrt(np.mean((cl - cl.mean()) ** 2))),
+           "cl_amplitude": float((cl.max() - cl.min()) / 2.0)}
+    z = cl - cl.mean()
+    up = [t[k] - z[k] * (t[k + 1] - t[k]) / (z[k + 1] - z[k]) for k in range(len(z) - 1) if z[k] < 0.0 <= z[k + 1]]
+    if len(up) >= 3:
+        period = (up[-1] - up[0]) / (len(up) - 1)  # in steps
+        out["strouhal"] = float(p.ref_L / (p.ref_U * period * time_step(p)))
+        out["shedding_periods"] = len(up) - 1
+    return out
+
+
+def time_step(p):
+    """Dt = Dx^2 nu_lattice / nu_p (Grid.cpp:1255-1258), omega as params.h sets it."""
+    from life_amd import config as cfg
+    Dx = p.height_p / (p.Ny - 1)
+    omega = p.omega
+    if omega <= 0:
+        cs = 1.0 / 3.0 ** 0.5
+        omega = 1.0 / (p.nu_p * p.tStep / ((cs * cs) * (Dx * Dx)) + 0.5)
+    return Dx * Dx * cfg.lattice_viscosity(omega) / p.nu_p
 
 
 def bench_config(args):
@@ -303,13 +335,25 @@ def bench_config(args):
     p, g = getattr(cfg, builder)()
     N = p.Nx * p.Ny
     runs = {}
+    # bodies: the force coefficients are sampled about 1000 times over the
+    # run (one host synchronisation per sample)
+    chunk = max(1, steps // 1000)
     for name, prec in (("f64", lib.LIFE_FP64), ("f32", lib.LIFE_FP32_SHIFTED)):
         c = Case(p, g, precision=prec)
+        series = []
         t0 = time.perf_counter()
-        c.step(steps)
+        if c.n_markers:
+            done = 0
+            while done < steps:
+                k = min(chunk, steps - done)
+                c.step(k)
+                done += k
+                series.append([done] + list(c.total_force()))
+        else:
+            c.step(steps)
         wall = time.perf_counter() - t0
         _, rho, u, _ = c.fields()
-        r = {"mlups": N * steps / wall / 1e6, "wall_s": wall, "rho": rho, "u": u}
+        r = {"mlups": N * steps / wall / 1e6, "wall_s": wall, "rho": rho, "u": u, "series": np.array(series)}
         if c.n_markers:
             r["force"] = c.total_force()
             r["tip"] = c.markers()["pos"].reshape(-1, 2)[-1].copy()
@@ -326,6 +370,8 @@ def bench_config(args):
         res["centreline_max_abs_f32_vs_f64"] = float(max(np.abs(a["u"][m, :, 0] - b["u"][m, :, 0]).max(),
                                                        np.abs(a["u"][:, m, 1] - b["u"][:, m, 1]).max()))
     if "force" in b:
+        res["coefficients_f64"] = coefficient_summary(b["series"], p)
+        res["coefficients_f32"] = coefficient_summary(a["series"], p)
         res["force_f64"] = b["force"].tolist()
         res["force_rel_err_f32"] = float(np.linalg.norm(a["force"][:2] - b["force"][:2]) /
                                          np.linalg.norm(b["force"][:2]))

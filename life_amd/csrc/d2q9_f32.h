@@ -14,6 +14,20 @@ namespace life {
 
 LIFE_DF float w32(int v) { return v == 0 ? 4.0f / 9.0f : (v < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
 
+// x * w_v without the weights' FP32 rounding bias.  w_v = k_v / 36 with
+// k_v = 16, 4, 1, and float(1/36) is 7.5e-9 (relative) above 1/36, the
+// same for every v (the k_v are powers of two).  Multiplying by it makes
+// every equilibrium 7.5e-9 too large, so each collision adds that fraction
+// of the site's mass and momentum: a body force proportional to u that
+// accumulates over thousands of steps.  Here 1/36 = kW36hi + kW36lo to
+// about 2^-50, k_v * x is exact, and one fused multiply-add rounds once.
+constexpr float kW36hi = 1.0f / 36.0f;
+constexpr float kW36lo = (float)(1.0 / 36.0 - (double)(1.0f / 36.0f));
+LIFE_DF float wmul(int v, float x) {
+    const float t = (v == 0 ? 16.0f : (v < 5 ? 4.0f : 1.0f)) * x;
+    return __builtin_fmaf(t, kW36hi, t * kW36lo);
+}
+
 // BGK + Guo on deviations.
 LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rho, float ux, float uy, float Fx,
                            float Fy, float go[kNV]) {
@@ -23,8 +37,8 @@ LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rh
     for (int v = 0; v < kNV; v++) {
         const float cx = (float)kCX[v], cy = (float)kCY[v];
         const float cu = cx * ux + cy * uy;
-        const float eq = w32(v) * (drho + rho * (3.0f * cu + 4.5f * cu * cu - usq));
-        const float fv = 3.0f * w32(v) * ((cx - ux) * Fx + (cy - uy) * Fy + 3.0f * cu * (cx * Fx + cy * Fy));
+        const float eq = wmul(v, drho + rho * (3.0f * cu + 4.5f * cu * cu - usq));
+        const float fv = wmul(v, 3.0f * ((cx - ux) * Fx + (cy - uy) * Fy + 3.0f * cu * (cx * Fx + cy * Fy)));
         go[v] = g[v] + omega * (eq - g[v]) + cF * fv;
     }
 }
@@ -90,9 +104,10 @@ LIFE_DF void collide_cm32(float omega, float cs2, float cs4, const float g[kNV],
         const float cx = (float)kCX[v], cy = (float)kCY[v];
         const float bx = 9.0f * (cx * cx) * XX + 6.0f * cx * X - 3.0f * XX;
         const float by = 9.0f * (cy * cy) * YY + 6.0f * cy * Y - 3.0f * YY;
-        const float dev = w32(v) * (0.5f * (bx + by) + 0.25f * bx * by);  // feqCM(1,u) - w
-        const float feq1 = w32(v) + dev;
-        go[v] = dev + drho * feq1 + m1[v] * k1 + m2[v] * k2 + m4[v] * k4 + m5[v] * k5 + m6[v] * k6 + m7[v] * k7;
+        const float dev = wmul(v, 0.5f * (bx + by) + 0.25f * bx * by);  // feqCM(1,u) - w
+        // drho * feqCM(1,u) = drho * w + drho * dev
+        go[v] = dev + wmul(v, drho) + drho * dev + m1[v] * k1 + m2[v] * k2 + m4[v] * k4 + m5[v] * k5 + m6[v] * k6 +
+                m7[v] * k7;
     }
 }
 
