@@ -87,15 +87,23 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
         return;
     }
     if (b.type == 0) return;
-    double rn, uxn, uyn;
-    if (explicit_state) {
-        rn = rho0[id];
-        uxn = u0[2 * id];
-        uyn = u0[2 * id + 1];
-    } else {
-        double fp[kNV];
-        load_site(src, S, id, fp);
-        moments64(fp, g.Fxm + fibm[id], g.Fym + fibm[S + id], rn, uxn, uyn);
+    // rho_n/u_n of the previous step outlive applyBCs + regularisedBC only
+    // at a pressure corner: everywhere else all three are overwritten below
+    // (wall/velocity set u, free-slip and pressure take u from the normal
+    // condition and the zero gradient, rho comes from the post-stream
+    // populations, the corner extrapolation or rho_in).  The previous
+    // populations, nine scattered loads per site, are read only there.
+    double rn = 0.0, uxn = 0.0, uyn = 0.0;
+    if (b.type == 4 && b.nx != 0 && b.ny != 0) {
+        if (explicit_state) {
+            rn = rho0[id];
+            uxn = u0[2 * id];
+            uyn = u0[2 * id + 1];
+        } else {
+            double fp[kNV];
+            load_site(src, S, id, fp);
+            moments64(fp, g.Fxm + fibm[id], g.Fym + fibm[S + id], rn, uxn, uyn);
+        }
     }
     const long long id1 = (long long)(b.i + b.nx) * Ny + (j + b.ny);
     const long long id2 = (long long)(b.i + 2 * b.nx) * Ny + (j + 2 * b.ny);
