@@ -6,6 +6,8 @@
 // Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "d2q9_f64.h"
 #include "launch.h"
 
@@ -502,11 +504,12 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
 // k_eps_far_check) and, if the system is banded, the solve by wavefront 0
 // (as k_eps_band_solve).  Same operations in the same order as those
 // kernels.  LDS: n * (3KB + 5) doubles; bodies of up to kFusedMaxMarkers
-// markers.  Used by the serving engine, whose coupling phase runs alone on
-// the device: a 1024-lane workgroup needs a CU's wave slots free at once, so
-// beside a concurrent bulk stream-collide (the single-case step schedule) it
-// waits for that kernel to drain and the separate small-block kernels are
-// faster there.
+// markers.  Used by the serving engine.  A large workgroup needs that many
+// of a CU's wave slots free at once, so beside a concurrent bulk
+// stream-collide (the single-case step schedule, and the serving engine's
+// early bulk kernel) it waits for CUs to drain; the serving engine launches
+// it with kFusedThreads lanes, and the single-case step uses the separate
+// small-block kernels.
 constexpr int kFusedMaxMarkers = 256;
 
 template <int KB>
@@ -582,10 +585,14 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     eps_band_solve_bw<KB, true, true>(*bw_sh, first, n, Bl, yl, m.eps, xb);
 }
 
-constexpr int kFusedThreads = 1024;
+// Lanes per body workgroup (at least one wavefront for the solve).  Beside
+// the early bulk kernel, 256, 512 and 1024 measured within 3% of each other
+// on MI355X and 128 8% slower (environment LIFE_SERVE_FUSED_THREADS = 64 |
+// 128 | 256 | 512 | 1024 for sweeps).
+constexpr int kFusedThreads = 512;
 
 template <int KB>
-__global__ __launch_bounds__(kFusedThreads) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
+__global__ __launch_bounds__(1024) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
                                                               const CaseSys* __restrict__ sys) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
@@ -922,7 +929,12 @@ bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int 
                                   hipStream_t s) {
     if (max_n > kFusedMaxMarkers) return false;
     if (nsys <= 0) return true;
-    k_recompute_body_batch<8><<<(unsigned)nsys, kFusedThreads, fused_recompute_lds(max_n), s>>>(cases, sys);
+    static const int nt = [] {
+        const char* e = std::getenv("LIFE_SERVE_FUSED_THREADS");
+        const int v = e ? std::atoi(e) : kFusedThreads;
+        return v == 64 || v == 128 || v == 512 || v == 1024 ? v : kFusedThreads;
+    }();
+    k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n), s>>>(cases, sys);
     k_eps_outside_gather_batch<<<(unsigned)((nsys + 255) / 256), 256, 0, s>>>(cases, sys, nsys, outside_out);
     return true;
 }

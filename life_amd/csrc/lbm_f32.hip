@@ -111,7 +111,9 @@ template <bool CM>
 __global__ __launch_bounds__(kServeBlock32) void k_push32_batch(const CaseDev* __restrict__ cases,
                                                                 const CaseBlock* __restrict__ blocks) {
     __shared__ PushStage<float, kServeBlock32> st;
-    const CaseBlock b = blocks[blockIdx.x];
+    // neighbouring blocks of a case on one XCD (the L2 that merges their
+    // shared edge lines)
+    const CaseBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const CaseDev& c = cases[b.c];
     const long long base = (long long)b.blk * kServeBlock32;
     if (c.explicit_state) {

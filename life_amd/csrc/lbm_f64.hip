@@ -109,7 +109,9 @@ template <bool CM>
 __global__ __launch_bounds__(kServeBlock64) void k_push64_batch(const CaseDev* __restrict__ cases,
                                                              const CaseBlock* __restrict__ blocks) {
     __shared__ PushStage<double> st;
-    const CaseBlock b = blocks[blockIdx.x];
+    // neighbouring blocks of a case on one XCD (the L2 that merges their
+    // shared edge lines)
+    const CaseBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const CaseDev& c = cases[b.c];
     const long long base = (long long)b.blk * kServeBlock64;
     const double* src = static_cast<const double*>(c.src);
