@@ -188,7 +188,12 @@ def serve_accuracy(pre, pinfo, pout, args):
     mx = lambda v, i: max(x[i] for x in v)
     return {"full_horizon_steps": pre[0]["steps"], "cases": len(full),
             "fp32_vs_fp64": {"max_tip_err_over_L": mx(full, 0), "max_abs_dC": mx(full, 1)},
-            "fp64_perturbed_1e-12_vs_fp64": {"cases": m, "max_tip_err_over_L": mx(pert, 0), "max_abs_dC": mx(pert, 1)},
+            # like for like with the perturbed runs: the same first m cases
+            "fp32_vs_fp64_first_cases": {"cases": m, "max_tip_err_over_L": mx(full[:m], 0),
+                                         "max_abs_dC": mx(full[:m], 1),
+                                         "tip_err_over_L": [x[0] for x in full[:m]]},
+            "fp64_perturbed_1e-12_vs_fp64": {"cases": m, "max_tip_err_over_L": mx(pert, 0), "max_abs_dC": mx(pert, 1),
+                                             "tip_err_over_L": [x[0] for x in pert]},
             "short_horizon": {"steps": H, "cases": m, "fp32_vs_fp64_max_tip_err_over_L": mx(short, 0),
                               "fp32_vs_fp64_max_abs_dC": mx(short, 1)}}
 
