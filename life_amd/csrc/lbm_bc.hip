@@ -74,8 +74,6 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
     const BCSite b = a.sites[k];
     const long long S = g.S, id = b.id;
     const int Ny = g.Ny, j = b.j;
-    double fs[kNV];
-    load_site(dst, S, id, fs);
     if (b.type == 5) {  // convective: f[2,6,8] = f_n + 3 w (delU . c)
         const double dux = a.delU[2 * j], duy = a.delU[2 * j + 1];
         const int vs[3] = {2, 6, 8};
@@ -87,6 +85,17 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
         return;
     }
     if (b.type == 0) return;
+    // the populations regularisedBC rebuilds from the others (the ones that
+    // entered from outside the domain) are never read, so only the known
+    // ones are loaded: six of nine on an edge
+    const bool corner = b.nx != 0 && b.ny != 0;
+    double fs[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++) {
+        const bool unknown = corner ? (kCX[v] == b.nx || kCY[v] == b.ny)
+                                    : ((b.nd == 0 ? kCX[v] : kCY[v]) == (b.nd == 0 ? b.nx : b.ny));
+        fs[v] = unknown ? 0.0 : Pops<T>::get(dst, S, v, id);
+    }
     // rho_n/u_n of the previous step outlive applyBCs + regularisedBC only
     // at a pressure corner: everywhere else all three are overwritten below
     // (wall/velocity set u, free-slip and pressure take u from the normal
@@ -94,7 +103,7 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
     // populations, the corner extrapolation or rho_in).  The previous
     // populations, nine scattered loads per site, are read only there.
     double rn = 0.0, uxn = 0.0, uyn = 0.0;
-    if (b.type == 4 && b.nx != 0 && b.ny != 0) {
+    if (b.type == 4 && corner) {
         if (explicit_state) {
             rn = rho0[id];
             uxn = u0[2 * id];
