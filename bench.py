@@ -40,7 +40,8 @@ def workload(name):
     if name == "canopy8192":
         p, g = cfg.canopy()
         return p, g, dict(workload="canopy_8192x4096_bgk", Nx=p.Nx, Ny=p.Ny, filaments=64, filament_length=200,
-                          fem_elements=50, markers_per_filament=201, tau=0.6, u_in=0.05, seed=20261202)
+                          fem_elements=50, markers_per_filament=201, filament_thickness=4.0, density_ratio=50.0,
+                          tau=0.6, u_in=0.05, seed=20261202)
     if name == "canopy8192_rigidfree":
         p, g = cfg.canopy(n_filaments=0)
         return p, g, dict(workload="channel_8192x4096_bgk_no_bodies", Nx=p.Nx, Ny=p.Ny, tau=0.6, u_in=0.05)

@@ -175,13 +175,18 @@ def flexible_fsi(Nx=2048, Ny=410, D=80.0, centre=(200.0, 200.0), L=280.0, h=16.0
 
 
 def canopy(Nx=8192, Ny=4096, n_filaments=64, L=200.0, n_el=50, min_spacing=60.0, tau=0.6, u_in=0.05,
-           h=1.0, density_ratio=10.0, kb_range=(1e-3, 1e-1), seed=20261202, x_range=None,
+           h=4.0, density_ratio=50.0, kb_range=(1e-3, 1e-1), seed=20261202, x_range=None,
            ramp_steps=2000.0):
     """Bandwidth-roofline canopy: BGK, uniform inlet, convective outlet,
     no-slip bottom, free-slip top (as the reference's Honami case), and
     n_filaments clamped vertical flexible filaments rooted on the bottom wall
     at seeded random x positions with a minimum spacing.  Bending stiffness
-    K_B = E I / (rho_f U^2 L^3), log-uniform; I = Dx h^3 / 12."""
+    K_B = E I / (rho_f U^2 L^3), log-uniform; I = Dx h^3 / 12.  Thickness
+    and density follow the Honami filaments (examples/Honami: density 50
+    against fluid 1, thickness 0.02 of the length): mass ratio
+    rho_s h / (rho_f L) = 1.  (With h = 1 and density 10, mass ratio 0.05,
+    the coupling of the most flexible filaments stops converging within 20
+    subiterations near step 9 370.)"""
     rng = random.Random(seed)
     lo, hi = x_range if x_range else (0.1 * Nx, 0.6 * Nx)
     # uniform random positions with a minimum gap: sorted uniform samples on
