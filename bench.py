@@ -457,9 +457,16 @@ def main():
     u32 = None
     if not args.no_accuracy and args.precision == "f32":
         _, _, u32, _ = case.fields()
+    # four more timed regions of the same length (after the fields for the
+    # accuracy check were taken): the spread of the full-step rate and of
+    # the kernel average, reported beside the contract's single region
+    reps = [(mlups, k_avg_s * 1e3)]
+    for _ in range(4):
+        t1 = time.perf_counter()
+        _, mk = case.step_timed(args.steps)
+        reps.append((N * args.steps / (time.perf_counter() - t1) / 1e6, mk / args.steps))
     # the same kernel with nothing beside it: a few steps without the
-    # overlap of the next step's stream-collide and the coupling (after the
-    # fields for the accuracy check were taken)
+    # overlap of the next step's stream-collide and the coupling
     alone_steps = min(args.steps, 20)
     saved = os.environ.get("LIFE_OVERLAP")
     os.environ["LIFE_OVERLAP"] = "0"
@@ -512,6 +519,9 @@ def main():
                                "frac": algo_bytes / k_alone_s / 1e9 / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "repeats": {"n": len(reps), "mlups": [r[0] for r in reps], "kernel_ms": [r[1] for r in reps],
+                    "median_mlups": float(np.median([r[0] for r in reps])),
+                    "median_kernel_ms": float(np.median([r[1] for r in reps]))},
     }
     print(json.dumps(out))
 
