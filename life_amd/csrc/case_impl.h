@@ -19,18 +19,23 @@ namespace life {
 // and from the device every step: the copies are then DMA transfers that
 // do not block the host, so the host must not write such an array while a
 // copy out of it may still be queued.
+//
+// hipHostMalloc and hipHostFree cost far more than the copies they serve,
+// and a serving engine creates and retires cases all the time, mostly of a
+// few marker counts; freed blocks are therefore kept, by size, for the next
+// array of that size (pinned_take / pinned_give, runtime.cpp), up to a cap,
+// and stay allocated until the process ends.
+void* pinned_take(size_t bytes);
+void pinned_give(void* p, size_t bytes);
+
 template <typename T>
 struct PinnedAllocator {
     using value_type = T;
     PinnedAllocator() = default;
     template <typename U>
     PinnedAllocator(const PinnedAllocator<U>&) {}
-    T* allocate(size_t n) {
-        void* p = nullptr;
-        if (hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
-        return static_cast<T*>(p);
-    }
-    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    T* allocate(size_t n) { return static_cast<T*>(pinned_take((n ? n : 1) * sizeof(T))); }
+    void deallocate(T* p, size_t n) { pinned_give(p, (n ? n : 1) * sizeof(T)); }
     template <typename U>
     bool operator==(const PinnedAllocator<U>&) const { return true; }
     template <typename U>

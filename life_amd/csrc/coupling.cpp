@@ -147,6 +147,7 @@ void finalise_markers(life_case* c) {
             }
         }
     for (auto& b : c->bodies) b.n = 0;
+    c->pos.reserve(2 * (size_t)nm);  // one pinned allocation, not one per growth
     for (int a = 0; a < nm; a++) {
         if (erase[a]) continue;
         Body& b = c->bodies[B[a]];

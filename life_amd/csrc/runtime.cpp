@@ -11,9 +11,57 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
 #include <stdexcept>
+#include <vector>
 
 namespace life {
+
+namespace {
+// Freed pinned blocks by size (never destroyed: the blocks stay allocated
+// until the process ends, and no HIP call runs from static destructors).
+struct PinnedCache {
+    std::mutex mu;
+    std::map<size_t, std::vector<void*>> blocks;
+    size_t bytes = 0;
+};
+PinnedCache& pinned_cache() {
+    static PinnedCache* c = new PinnedCache();
+    return *c;
+}
+constexpr size_t kPinnedCacheMax = size_t(256) << 20;
+}  // namespace
+
+void* pinned_take(size_t bytes) {
+    {
+        PinnedCache& pc = pinned_cache();
+        std::lock_guard<std::mutex> lock(pc.mu);
+        auto it = pc.blocks.find(bytes);
+        if (it != pc.blocks.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            pc.bytes -= bytes;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    return p;
+}
+
+void pinned_give(void* p, size_t bytes) {
+    if (!p) return;
+    PinnedCache& pc = pinned_cache();
+    {
+        std::lock_guard<std::mutex> lock(pc.mu);
+        if (pc.bytes + bytes <= kPinnedCacheMax) {
+            pc.blocks[bytes].push_back(p);
+            pc.bytes += bytes;
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
 
 static thread_local std::string g_error;
 
@@ -21,6 +69,100 @@ void set_error(const std::string& msg) { g_error = msg; }
 
 void dfree(void* p) {
     if (p) (void)hipFree(p);
+}
+
+namespace {
+struct DeviceCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void*>> blocks;  // (device, bytes) -> free blocks
+    std::map<void*, std::pair<int, size_t>> live;                // blocks handed out
+    size_t bytes = 0;
+};
+DeviceCache& device_cache() {
+    static DeviceCache* c = new DeviceCache();  // never destroyed, like the blocks it keeps
+    return *c;
+}
+constexpr size_t kDeviceCacheMax = size_t(24) << 30;
+
+// Return every kept block of `device` (caller holds the lock).
+void release_device_blocks(DeviceCache& dc, int device) {
+    for (auto it = dc.blocks.begin(); it != dc.blocks.end(); ++it) {
+        if (it->first.first != device) continue;
+        for (void* p : it->second) {
+            (void)hipFree(p);
+            dc.bytes -= it->first.second;
+        }
+        it->second.clear();
+    }
+}
+}  // namespace
+
+// Grid streams of destroyed grids, per device, handed to the next grid
+// (creating a stream costs more than a small case's step).
+std::mutex g_stream_mu;
+std::map<int, std::vector<hipStream_t>> g_free_streams;
+
+hipStream_t take_grid_stream(int device) {
+    {
+        std::lock_guard<std::mutex> lock(g_stream_mu);
+        std::vector<hipStream_t>& v = g_free_streams[device];
+        if (!v.empty()) {
+            hipStream_t st = v.back();
+            v.pop_back();
+            return st;
+        }
+    }
+    hipStream_t st = nullptr;
+    LIFE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    return st;
+}
+
+// The stream has no queued work (synchronised by the caller).
+void give_grid_stream(int device, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    g_free_streams[device].push_back(st);
+}
+
+void* dalloc_cached_bytes(size_t bytes) {
+    int dev = 0;
+    LIFE_HIP(hipGetDevice(&dev));
+    DeviceCache& dc = device_cache();
+    std::lock_guard<std::mutex> lock(dc.mu);
+    auto it = dc.blocks.find({dev, bytes});
+    if (it != dc.blocks.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        dc.bytes -= bytes;
+        dc.live[p] = {dev, bytes};
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        release_device_blocks(dc, dev);
+        LIFE_HIP(hipMalloc(&p, bytes));
+    }
+    dc.live[p] = {dev, bytes};
+    return p;
+}
+
+void dfree_cached(void* p) {
+    if (!p) return;
+    DeviceCache& dc = device_cache();
+    std::lock_guard<std::mutex> lock(dc.mu);
+    auto it = dc.live.find(p);
+    if (it == dc.live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const std::pair<int, size_t> key = it->second;
+    dc.live.erase(it);
+    if (dc.bytes + key.second <= kDeviceCacheMax) {
+        dc.blocks[key].push_back(p);
+        dc.bytes += key.second;
+    } else {
+        (void)hipFree(p);
+    }
 }
 
 CaseScalars derive_scalars(const life_params& p) {
@@ -434,7 +576,7 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         if ((long long)p->Nx * p->Ny > (1LL << 31) - 1) throw std::runtime_error("lattice too large for 32-bit site ids");
         g->device = device;
         LIFE_HIP(hipSetDevice(device));
-        LIFE_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        g->stream = take_grid_stream(device);
         LIFE_HIP(hipEventCreate(&g->ev_a));
         LIFE_HIP(hipEventCreate(&g->ev_b));
         LIFE_HIP(hipEventCreate(&g->ev_k0));
@@ -521,10 +663,11 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         // device buffers
         const size_t S = (size_t)s.S;
         for (int b = 0; b < 2; b++)
-            g->f[b] = precision == LIFE_FP64 ? (void*)dalloc<double>(9 * S) : (void*)dalloc<float>(9 * S);
-        g->rho0 = dalloc<double>(S);
-        g->u0 = dalloc<double>(2 * S);
-        g->fibm = dalloc<double>(2 * S);
+            g->f[b] = precision == LIFE_FP64 ? (void*)dalloc_cached<double>(9 * S)
+                                             : (void*)dalloc_cached<float>(9 * S);
+        g->rho0 = dalloc_cached<double>(S);
+        g->u0 = dalloc_cached<double>(2 * S);
+        g->fibm = dalloc_cached<double>(2 * S);
         g->bc = dalloc<BCSite>(g->nbc);
         g->u_in = dalloc<double>(2 * Ny);
         g->rho_in = dalloc<double>(Ny);
@@ -572,15 +715,16 @@ int life_grid_destroy(life_grid* g) {
         if (g->ev_bulk) (void)hipEventSynchronize(g->ev_bulk);  // this grid's last bulk kernel
         if (g->ev_tail) (void)hipEventSynchronize(g->ev_tail);  // and step tail on stream2
         free_markers(g);
-        for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm, (void*)g->bc,
-                        (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA,
+        // the grid's work is done (synchronised above): its lattice blocks can be handed out again
+        for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm}) dfree_cached(p);
+        for (void* p : {(void*)g->bc, (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA,
                         (void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
                         (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
         if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
         for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk, g->ev_tail})
             if (e) (void)hipEventDestroy(e);
-        if (g->stream) (void)hipStreamDestroy(g->stream);
+        if (g->stream) give_grid_stream(g->device, g->stream);
         delete g;
     });
 }

@@ -43,6 +43,17 @@ T* dalloc(size_t n) {
     return static_cast<T*>(p);
 }
 void dfree(void* p);
+// The lattice-sized buffers of a grid (populations, rho0/u0, force_ibm):
+// a freed block is kept, by device and size, for the next grid of that
+// size (a serving engine creates and retires grids all the time, mostly of
+// a few sizes), up to a cap; dfree_cached takes only blocks that no queued
+// work uses any more.  A failed allocation first returns the kept blocks.
+void* dalloc_cached_bytes(size_t bytes);
+void dfree_cached(void* p);
+template <typename T>
+T* dalloc_cached(size_t n) {
+    return static_cast<T*>(dalloc_cached_bytes((n ? n : 1) * sizeof(T)));
+}
 
 }  // namespace life
 
