@@ -29,7 +29,7 @@ LIFE_DF float wmul(int v, float x) {
 }
 
 // BGK + Guo on deviations.
-LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rho, float ux, float uy, float Fx,
+LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float ux, float uy, float Fx,
                            float Fy, float go[kNV]) {
     const float usq = 1.5f * (ux * ux + uy * uy);
     const float cF = 1.0f - 0.5f * omega;
@@ -37,7 +37,10 @@ LIFE_DF void collide_bgk32(float omega, const float g[kNV], float drho, float rh
     for (int v = 0; v < kNV; v++) {
         const float cx = (float)kCX[v], cy = (float)kCY[v];
         const float cu = cx * ux + cy * uy;
-        const float eq = wmul(v, drho + rho * (3.0f * cu + 4.5f * cu * cu - usq));
+        // rho * P as P + drho * P: 1 + drho rounded to float would carry
+        // the same error over every site of a region of uniform density
+        const float P = 3.0f * cu + 4.5f * cu * cu - usq;
+        const float eq = wmul(v, drho + P + drho * P);
         const float fv = wmul(v, 3.0f * ((cx - ux) * Fx + (cy - uy) * Fy + 3.0f * cu * (cx * Fx + cy * Fy)));
         go[v] = g[v] + omega * (eq - g[v]) + cF * fv;
     }
@@ -115,14 +118,21 @@ template <bool CM>
 LIFE_DF void collide32(const GridScalars& g, const float gs[kNV], float drho, float ux, float uy, float Fx, float Fy,
                        float go[kNV]) {
     if (CM) collide_cm32((float)g.omega, (float)g.cs2, (float)g.cs4, gs, drho, ux, uy, Fx, Fy, go);
-    else collide_bgk32((float)g.omega, gs, drho, 1.0f + drho, ux, uy, Fx, Fy, go);
+    else collide_bgk32((float)g.omega, gs, drho, ux, uy, Fx, Fy, go);
 }
 
 LIFE_DF void moments32(const float gs[kNV], float Fx, float Fy, float& drho, float& ux, float& uy) {
     const float dr = ((gs[0] + gs[1]) + (gs[2] + gs[3])) + ((gs[4] + gs[5]) + (gs[6] + gs[7])) + gs[8];
     const float jx = (gs[1] - gs[2]) + (gs[5] - gs[6]) + (gs[7] - gs[8]);
     const float jy = (gs[3] - gs[4]) + (gs[5] - gs[6]) + (gs[8] - gs[7]);
-    const float inv = 1.0f / (1.0f + dr);
+    // 1 / (1 + dr) with the rounding error e of the sum 1 + dr put back
+    // (|dr| < 1, so e is exact): a bare 1 + dr carries the same error over
+    // every site of a region of uniform density, a velocity bias of up to
+    // 6e-8 shared by the whole region
+    const float r = 1.0f + dr;
+    const float e = dr - (r - 1.0f);
+    const float inv0 = 1.0f / r;
+    const float inv = inv0 - e * inv0 * inv0;
     drho = dr;
     ux = (jx + 0.5f * Fx) * inv;
     uy = (jy + 0.5f * Fy) * inv;
