@@ -604,39 +604,63 @@ __global__ __launch_bounds__(1024) void k_recompute_body_batch(const CaseDev* __
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
 // the supports (GridClass::macroscopic values: force_xy only) interpolated
-// with the support deltas, then F = 2 (Dt/Dx rho_I U - (rho u)_I).
+// with the support deltas, then F = 2 (Dt/Dx rho_I U - (rho u)_I).  One
+// lane per (marker, support slot): each lane takes its support's moments
+// and weighted terms, and the marker's first lane adds them up in support
+// order, as the reference's loop does, so the nine gathers of nine
+// populations are in flight at once.  kInterpMarkers markers per workgroup
+// of kSupp lanes each.
+constexpr int kInterpMarkers = 28;
+constexpr int kInterpThreads = kInterpMarkers * kSupp;  // 252
+
 template <typename T>
-__device__ inline void interpolate_one(const T* __restrict__ f, const MarkerDev& m, const GridScalars& g, int k) {
-    double iR = 0.0, mx = 0.0, my = 0.0;
-    const int cnt = m.supp_cnt[k];
-    for (int s = 0; s < cnt; s++) {
+__device__ inline void interpolate_slots(const T* __restrict__ f, const MarkerDev& m, const GridScalars& g,
+                                         int k0, double* __restrict__ sh) {
+    const int t = threadIdx.x;
+    const int km = t / kSupp, s = t - km * kSupp;
+    const int k = k0 + km;
+    double* a = sh;
+    double* bx = sh + kInterpThreads;
+    double* by = sh + 2 * kInterpThreads;
+    if (t < kInterpThreads && k < m.n && s < m.supp_cnt[k]) {
         const long long id = m.supp_site[k * kSupp + s];
         double fs[kNV], rho, ux, uy;
         load_site(f, g.S, id, fs);
         moments64(fs, g.Fx, g.Fy, rho, ux, uy);
         const double d = m.supp_delta[k * kSupp + s];
-        iR += rho * d * 1.0 * 1.0;
-        mx += rho * ux * d * 1.0 * 1.0;
-        my += rho * uy * d * 1.0 * 1.0;
+        a[t] = rho * d * 1.0 * 1.0;
+        bx[t] = rho * ux * d * 1.0 * 1.0;
+        by[t] = rho * uy * d * 1.0 * 1.0;
     }
-    const double vs = g.velScale;
-    m.force[2 * k] = 2.0 * (vs * iR * m.vel[2 * k] - mx);
-    m.force[2 * k + 1] = 2.0 * (vs * iR * m.vel[2 * k + 1] - my);
+    __syncthreads();
+    if (t < kInterpThreads && s == 0 && k < m.n) {
+        double iR = 0.0, mx = 0.0, my = 0.0;
+        const int cnt = m.supp_cnt[k];
+        for (int q = 0; q < cnt; q++) {
+            iR += a[t + q];
+            mx += bx[t + q];
+            my += by[t + q];
+        }
+        const double vs = g.velScale;
+        m.force[2 * k] = 2.0 * (vs * iR * m.vel[2 * k] - mx);
+        m.force[2 * k + 1] = 2.0 * (vs * iR * m.vel[2 * k + 1] - my);
+    }
 }
 
 template <typename T>
-__global__ void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
+__global__ __launch_bounds__(256) void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
+    __shared__ double sh[3 * kInterpThreads];
     coupling_priority();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m.n) interpolate_one(f, m, g, k);
+    interpolate_slots(f, m, g, blockIdx.x * kInterpMarkers, sh);
 }
 
 // Batched: the populations after this step's stream-collide.
 template <typename T>
-__global__ void k_interpolate_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
+__global__ __launch_bounds__(256) void k_interpolate_batch(const CaseDev* __restrict__ cases,
+                                                           const int* __restrict__ list) {
+    __shared__ double sh[3 * kInterpThreads];
     const CaseDev& c = cases[list[blockIdx.y]];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < c.m.n) interpolate_one(static_cast<const T*>(c.dst), c.m, c.g, k);
+    interpolate_slots(static_cast<const T*>(c.dst), c.m, c.g, blockIdx.x * kInterpMarkers, sh);
 }
 
 // ---------------------------------------------------------------- spread
@@ -838,7 +862,7 @@ void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, 
 template <typename T>
 void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s) {
     if (m.n <= 0) return;
-    k_interpolate<T><<<(m.n + 127) / 128, 128, 0, s>>>(f, m, g);
+    k_interpolate<T><<<(m.n + kInterpMarkers - 1) / kInterpMarkers, kInterpThreads, 0, s>>>(f, m, g);
 }
 template void launch_interpolate<double>(const double*, const MarkerDev&, const GridScalars&, hipStream_t);
 template void launch_interpolate<float>(const float*, const MarkerDev&, const GridScalars&, hipStream_t);
@@ -891,7 +915,8 @@ void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, i
 template <typename T>
 void launch_interpolate_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s) {
     if (n <= 0 || max_markers <= 0) return;
-    k_interpolate_batch<T><<<dim3((unsigned)((max_markers + 127) / 128), (unsigned)n), 128, 0, s>>>(cases, list);
+    k_interpolate_batch<T><<<dim3((unsigned)((max_markers + kInterpMarkers - 1) / kInterpMarkers), (unsigned)n),
+                             kInterpThreads, 0, s>>>(cases, list);
 }
 template void launch_interpolate_batch<double>(const CaseDev*, const int*, int, int, hipStream_t);
 template void launch_interpolate_batch<float>(const CaseDev*, const int*, int, int, hipStream_t);
