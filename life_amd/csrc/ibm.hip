@@ -113,6 +113,18 @@ __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
     compute_ds_body(m, g, flexible_only, blockIdx.x, spos);
 }
 
+// Supports and spacing of one body per workgroup in one launch: the two
+// are independent (spacing reads only positions), so the coupling chain
+// has one kernel fewer to wait for.
+__global__ void k_support_ds(MarkerDev m, GridScalars g, int flexible_only) {
+    coupling_priority();
+    extern __shared__ double spos[];
+    const int b = blockIdx.x;
+    const int first = m.body_first[b], cnt = m.body_n[b];
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) find_support_one(m, g, flexible_only, first + k);
+    compute_ds_body(m, g, flexible_only, b, spos);
+}
+
 __global__ void k_compute_ds_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list, int flexible_only) {
     extern __shared__ double spos[];
     const CaseDev& c = cases[list[blockIdx.y]];
@@ -846,6 +858,11 @@ void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_fi
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
     if (m.n <= 0) return;
     k_find_support<<<(m.n + 127) / 128, 128, 0, s>>>(m, g, flexible_only);
+}
+
+void launch_support_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s) {
+    if (m.n <= 0 || nbodies <= 0) return;
+    k_support_ds<<<nbodies, 256, sizeof(double) * 2 * kDsLdsMarkers, s>>>(m, g, flexible_only);
 }
 
 void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s) {

@@ -82,6 +82,8 @@ struct SpreadScratch {
 };
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s);
+// launch_find_support + launch_compute_ds in one launch (one workgroup per body).
+void launch_support_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s);
 void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s);
 void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s);
 // Banded epsilon systems (one per entry of sys_first/sys_n), assembled and

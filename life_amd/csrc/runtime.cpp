@@ -384,8 +384,7 @@ void ibm_spread(life_grid* g) {
 
 void ibm_find_support(life_grid* g, int flexible_only) {
     if (g->md.n <= 0) return;
-    launch_find_support(g->md, g->g, flexible_only, g->stream);
-    launch_compute_ds(g->md, g->g, g->nbodies, flexible_only, g->stream);
+    launch_support_ds(g->md, g->g, g->nbodies, flexible_only, g->stream);
     LIFE_HIP(hipGetLastError());
 }
 
