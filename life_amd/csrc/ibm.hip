@@ -182,15 +182,6 @@ __device__ inline void eps_band_entry(const MarkerDev& m, const GridScalars& g, 
     band[t] = a;
 }
 
-__global__ void k_eps_band_entries(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
-                                   const int* __restrict__ sys_n, int KB, double* __restrict__ band,
-                                   const long long* __restrict__ band_off) {
-    coupling_priority();
-    const int sys = blockIdx.y;
-    eps_band_entry(m, g, sys_first[sys], sys_n[sys], KB, band + band_off[sys],
-                   (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 // Pairs outside the band that could couple (support of i within 1.5 of
 // x_i, delta of j within 1.5 of the support: |x_i - x_j| < 3 spacings).
 // One thread per pair r < c - KB (the test is symmetric); grid.y = system.
@@ -205,12 +196,21 @@ __device__ inline void eps_far_pair(const MarkerDev& m, const GridScalars& g, in
         *outside = 1;
 }
 
-__global__ void k_eps_far_check(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
-                                const int* __restrict__ sys_n, int KB, int* __restrict__ outside) {
+// Band entries (the first be workgroups of grid.x) and the out-of-band
+// check (the rest) of each system in one launch: the two are independent
+// and the solve needs both, so the coupling chain waits for one kernel.
+__global__ void k_eps_band_prep(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
+                                const int* __restrict__ sys_n, int KB, double* __restrict__ band,
+                                const long long* __restrict__ band_off, int* __restrict__ outside, int be) {
     coupling_priority();
     const int sys = blockIdx.y;
-    eps_far_pair(m, g, sys_first[sys], sys_n[sys], KB, outside + sys,
-                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    const int bx = (int)blockIdx.x;
+    if (bx < be)
+        eps_band_entry(m, g, sys_first[sys], sys_n[sys], KB, band + band_off[sys],
+                       (long long)bx * blockDim.x + threadIdx.x);
+    else
+        eps_far_pair(m, g, sys_first[sys], sys_n[sys], KB, outside + sys,
+                     (long long)(bx - be) * blockDim.x + threadIdx.x);
 }
 
 // Batched assembly: grid.y = entry of the system table.
@@ -512,8 +512,8 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
 // body per workgroup, without global round trips between the stages:
 // supports of its markers (to global memory, as k_find_support), spacing
 // from positions staged in LDS (as k_compute_ds), the band of its epsilon
-// system built in LDS (as k_eps_band_entries), the out-of-band check (as
-// k_eps_far_check) and, if the system is banded, the solve by wavefront 0
+// system built in LDS (as eps_band_entry), the out-of-band check (as
+// eps_far_pair) and, if the system is banded, the solve by wavefront 0
 // (as k_eps_band_solve).  Same operations in the same order as those
 // kernels.  LDS: n * (3KB + 5) doubles; bodies of up to kFusedMaxMarkers
 // markers.  Used by the serving engine.  A large workgroup needs that many
@@ -846,9 +846,9 @@ void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_fi
     for (int k = 0; k < nsys; k++) max_n = max_n > host_n[k] ? max_n : host_n[k];
     const int W = 3 * KB + 1;
     const long long ne = (long long)max_n * W, np = (long long)max_n * max_n;
-    k_eps_band_entries<<<dim3((unsigned)((ne + 255) / 256), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, band,
-                                                                                 band_off);
-    k_eps_far_check<<<dim3((unsigned)((np + 255) / 256), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, outside);
+    const int be = (int)((ne + 255) / 256), bf = (int)((np + 255) / 256);
+    k_eps_band_prep<<<dim3((unsigned)(be + bf), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, band, band_off, outside,
+                                                                     be);
     // band, then right-hand side, in LDS for systems of up to lds_rows rows
     const int rows = eps_lds_rows(KB, max_n);
     k_eps_band_solve<8><<<nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(sys_first, sys_n, outside, band,
