@@ -600,7 +600,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
 // Lanes per body workgroup (at least one wavefront for the solve).  Beside
 // the early bulk kernel, 256, 512 and 1024 measured within 3% of each other
 // on MI355X and 128 8% slower (environment LIFE_SERVE_FUSED_THREADS = 64 |
-// 128 | 256 | 512 | 1024 for sweeps).
+// 128 | 256 | 512 | 1024 for sweeps, 0 for the separate batched kernels).
 constexpr int kFusedThreads = 512;
 
 template <int KB>
@@ -974,8 +974,9 @@ bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int 
     static const int nt = [] {
         const char* e = std::getenv("LIFE_SERVE_FUSED_THREADS");
         const int v = e ? std::atoi(e) : kFusedThreads;
-        return v == 64 || v == 128 || v == 512 || v == 1024 ? v : kFusedThreads;
+        return v == 0 || v == 64 || v == 128 || v == 256 || v == 1024 ? v : kFusedThreads;
     }();
+    if (nt == 0) return false;  // the separate batched kernels
     k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n), s>>>(cases, sys);
     k_eps_outside_gather_batch<<<(unsigned)((nsys + 255) / 256), 256, 0, s>>>(cases, sys, nsys, outside_out);
     return true;
