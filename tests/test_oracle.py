@@ -4,7 +4,9 @@ The reference ships no golden data (its regression data is generated
 locally and git-ignored), so these checks are what makes the oracle
 trustworthy; a dropped term, a wrong sign or a transposed index in any of
 the restated formulas fails one of them."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -235,10 +237,9 @@ def test_convective_outlet_uniform_flow(oracle):
 
 
 # Ghia, Ghia & Shin (1982), Table I, Re = 100: u/U on the vertical centreline
-GHIA_Y = [1.0, 0.9766, 0.9688, 0.9609, 0.9531, 0.8516, 0.7344, 0.6172, 0.5, 0.4531, 0.2813, 0.1719,
-          0.1016, 0.0703, 0.0625, 0.0547, 0.0]
-GHIA_U = [1.0, 0.84123, 0.78871, 0.73722, 0.68717, 0.23151, 0.00332, -0.13641, -0.20581, -0.21090,
-          -0.15662, -0.10150, -0.06434, -0.04775, -0.04192, -0.03717, 0.0]
+_GHIA = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "ghia1982_re100_u_centreline.json")))
+GHIA_Y, GHIA_U = _GHIA["y"], _GHIA["u"]
 
 
 def test_lid_driven_cavity_ghia(oracle):
