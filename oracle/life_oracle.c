@@ -1100,6 +1100,19 @@ static void fem_newton_dynamic(orc_case *c, fem_t *F) {
     free(vec);
 }
 
+/* FEMBodyClass::subResidual, FEMBody.cpp:244-256: R_k = U - U_km1 (R_km1
+ * holds the previous subiteration's R_k), then |R_k|^2 and the numerator
+ * R_km1.(R_k - R_km1) and denominator |R_k - R_km1|^2 of the Aitken factor. */
+void orc_sub_residual(int nd, const double *U, const double *Ukm1, double *Rk, const double *Rkm1, double out[3]) {
+    for (int k = 0; k < nd; k++) Rk[k] = U[k] - Ukm1[k];
+    out[0] = vdot(Rk, Rk, nd);
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < nd; k++) num += Rkm1[k] * (Rk[k] - Rkm1[k]);
+    for (int k = 0; k < nd; k++) den += (Rk[k] - Rkm1[k]) * (Rk[k] - Rkm1[k]);
+    out[1] = num;
+    out[2] = den;
+}
+
 /* FEMBodyClass::dynamicFEM, FEMBody.cpp:26-68. */
 static void fem_dynamic(orc_case *c, fem_t *F) {
     int nd = F->ndof;
@@ -1117,15 +1130,12 @@ static void fem_dynamic(orc_case *c, fem_t *F) {
     } while (F->resNR > 1e-10 && F->itNR < 20);
     fem_finish_newmark(c, F);
     fem_update_ibm(c, F);
-    /* FEMBodyClass::subResidual, FEMBody.cpp:244-256 */
     double *t = F->Rkm1; F->Rkm1 = F->Rk; F->Rk = t;  /* R_km1 = R_k (old R_km1 content discarded) */
-    for (int k = 0; k < nd; k++) F->Rk[k] = F->U[k] - F->Ukm1[k];
-    F->subRes = vdot(F->Rk, F->Rk, nd);
-    double num = 0.0, den = 0.0;
-    for (int k = 0; k < nd; k++) num += F->Rkm1[k] * (F->Rk[k] - F->Rkm1[k]);
-    for (int k = 0; k < nd; k++) den += (F->Rk[k] - F->Rkm1[k]) * (F->Rk[k] - F->Rkm1[k]);
-    F->subNum = num;
-    F->subDen = den;
+    double out[3];
+    orc_sub_residual(nd, F->U, F->Ukm1, F->Rk, F->Rkm1, out);
+    F->subRes = out[0];
+    F->subNum = out[1];
+    F->subDen = out[2];
 }
 
 /* FEMBodyClass::staticFEM + newtonRaphsonStatic, FEMBody.cpp:448-484:
@@ -1379,6 +1389,26 @@ static int remove_overlap(orc_case *c) {
     return 0;
 }
 
+/* The matrix of ObjectsClass::computeEpsilon, Objects.cpp:258-302, for the
+ * markers nodes[0..dim): A_ij = ds_j * sum over the supports s of marker i
+ * of delta_i(s) * delta(x_j - s). */
+static void eps_matrix(const orc_case *c, const int *nodes, int dim, double *A) {
+    for (int i = 0; i < dim; i++) {
+        const marker_t *mi = &c->m[nodes[i]];
+        for (int j = 0; j < dim; j++) {
+            const marker_t *mj = &c->m[nodes[j]];
+            double a = 0.0;
+            for (int s = 0; s < mi->nsupp; s++) {
+                double dx = fabs(mj->pos[0] / c->Dx - mi->supp[s].i);
+                double dy = fabs(mj->pos[1] / c->Dx - mi->supp[s].j);
+                a += mi->supp[s].d * (orc_dirac_delta(dx) * orc_dirac_delta(dy));
+            }
+            a *= 1.0 * 1.0 * mj->ds;
+            A[(size_t)i * dim + j] = a;
+        }
+    }
+}
+
 /* ObjectsClass::computeEpsilon, Objects.cpp:235-321. */
 static int compute_epsilon(orc_case *c) {
     int nsys = c->p.uni_epsilon ? 1 : c->nb;
@@ -1399,21 +1429,8 @@ static int compute_epsilon(orc_case *c) {
         double *A = xcalloc((size_t)dim * dim, sizeof(double));
         double *rhs = xcalloc(dim, sizeof(double));
         double *x = xcalloc(dim, sizeof(double));
-        for (int i = 0; i < dim; i++) {
-            const marker_t *mi = &c->m[nodes[i]];
-            for (int j = 0; j < dim; j++) {
-                const marker_t *mj = &c->m[nodes[j]];
-                double a = 0.0;
-                for (int s = 0; s < mi->nsupp; s++) {
-                    double dx = fabs(mj->pos[0] / c->Dx - mi->supp[s].i);
-                    double dy = fabs(mj->pos[1] / c->Dx - mi->supp[s].j);
-                    a += mi->supp[s].d * (orc_dirac_delta(dx) * orc_dirac_delta(dy));
-                }
-                a *= 1.0 * 1.0 * mj->ds;
-                A[(size_t)i * dim + j] = a;
-            }
-            rhs[i] = 1.0;
-        }
+        eps_matrix(c, nodes, dim, A);
+        for (int i = 0; i < dim; i++) rhs[i] = 1.0;
         orc_lu_solve(dim, 0, A, rhs, x);
         for (int i = 0; i < dim; i++) c->m[nodes[i]].eps = x[i];
         free(A);
@@ -1422,6 +1439,18 @@ static int compute_epsilon(orc_case *c) {
         free(all);
     }
     return 0;
+}
+
+/* The relaxation factor of ObjectsClass::recomputeObjectVals,
+ * Objects.cpp:179-190: at the first relaxed subiteration the previous
+ * factor's sign with its magnitude capped at relaxMax, afterwards the Aitken
+ * update -relax * num / den (num, den from orc_sub_residual). */
+double orc_aitken_relax(int subIt, double relax, double relaxMax, double num, double den) {
+    if (subIt == 1) {
+        double r = fabs(relax) < relaxMax ? fabs(relax) : relaxMax;
+        return (double)(sgn(relax) * r);
+    }
+    return -relax * num / den;
 }
 
 /* ObjectsClass::recomputeObjectVals, Objects.cpp:152-232. */
@@ -1433,12 +1462,7 @@ static int recompute_object_vals(orc_case *c) {
                 fem_predictor(c, c->b[b].fem);
             }
     } else {
-        if (c->subIt == 1) {
-            double r = fabs(c->relax) < c->p.relaxMax ? fabs(c->relax) : c->p.relaxMax;
-            c->relax = (double)(sgn(c->relax) * r);
-        } else {
-            c->relax = -c->relax * c->subNum / c->subDen;
-        }
+        c->relax = orc_aitken_relax(c->subIt, c->relax, c->p.relaxMax, c->subNum, c->subDen);
         for (int b = 0; b < c->nb; b++)
             if (c->b[b].flex == 0) {
                 fem_t *F = c->b[b].fem;
@@ -1709,4 +1733,93 @@ void orc_total_force(const orc_case *c, double out[4]) {
     out[1] = F[1];
     out[2] = F[0] / ND;
     out[3] = F[1] / ND;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Unit-level access to the restated steps, so that each can be checked on   */
+/* its own against what the mathematics fixes.                               */
+/* ------------------------------------------------------------------------- */
+
+/* ObjectsClass::ibmKernelInterp, Objects.cpp:102-117, on the stored rho/u. */
+void orc_ibm_interpolate(orc_case *c) {
+    memset(c->fibm, 0, sizeof(double) * c->N * 2);
+    for (int n = 0; n < c->nm; n++) marker_interp_force(c, &c->m[n]);
+}
+
+/* ObjectsClass::ibmKernelSpread, Objects.cpp:120-149. */
+void orc_ibm_spread(orc_case *c) {
+    memset(c->fibm, 0, sizeof(double) * c->N * 2);
+    for (int n = 0; n < c->nm; n++) marker_spread(c, &c->m[n]);
+    for (int n = 0; n < c->nm; n++) marker_update_macro(c, &c->m[n]);
+}
+
+/* Overwrite marker values (NULL: keep). */
+void orc_set_markers(orc_case *c, const double *pos, const double *vel, const double *force, const double *eps,
+                     const double *ds) {
+    for (int n = 0; n < c->nm; n++) {
+        marker_t *m = &c->m[n];
+        if (pos) { m->pos[0] = pos[2 * n]; m->pos[1] = pos[2 * n + 1]; }
+        if (vel) { m->vel[0] = vel[2 * n]; m->vel[1] = vel[2 * n + 1]; }
+        if (force) { m->force[0] = force[2 * n]; m->force[1] = force[2 * n + 1]; }
+        if (eps) m->eps = eps[n];
+        if (ds) m->ds = ds[n];
+    }
+}
+
+/* findSupport + computeDs for every marker (Objects.cpp:949-955). */
+int orc_ibm_supports(orc_case *c) {
+    for (int n = 0; n < c->nm; n++) {
+        if (marker_support(c, &c->m[n])) return -1;
+        marker_ds(c, &c->m[n]);
+    }
+    return 0;
+}
+
+/* The epsilon system `sys` (UNI_EPSILON: the one system of all markers;
+ * else body sys): its matrix into A (dim x dim, row-major, may be NULL).
+ * Returns dim. */
+int orc_epsilon_matrix(const orc_case *c, int sys, double *A) {
+    if (c->p.uni_epsilon) {
+        int dim = c->nm;
+        if (A) {
+            int *all = xcalloc(dim, sizeof(int));
+            for (int k = 0; k < dim; k++) all[k] = k;
+            eps_matrix(c, all, dim, A);
+            free(all);
+        }
+        return dim;
+    }
+    if (sys < 0 || sys >= c->nb) return -1;
+    if (A) eps_matrix(c, c->b[sys].nodes, c->b[sys].n, A);
+    return c->b[sys].n;
+}
+
+/* Interpolated density and momentum of every marker from the last
+ * interpolation (IBMNode.h:56-57). */
+void orc_get_interp(const orc_case *c, double *rho, double *mom) {
+    for (int n = 0; n < c->nm; n++) {
+        rho[n] = c->m[n].interpRho;
+        mom[2 * n] = c->m[n].interpMom[0];
+        mom[2 * n + 1] = c->m[n].interpMom[1];
+    }
+}
+
+/* GridClass::getNormalVector at one site: n[2] and the normal direction. */
+int orc_normal_vector(const orc_case *c, int i, int j, int *n, int *nd) { return normal_vector(c, i, j, n, nd); }
+
+/* Convective outlet increments of the last lattice step (Ny x 2). */
+void orc_get_delu(const orc_case *c, double *delU) { memcpy(delU, c->delU, sizeof(double) * c->Ny * 2); }
+
+/* Inlet velocity profile in lattice units (Ny x 2). */
+void orc_get_inlet(const orc_case *c, double *u_in) { memcpy(u_in, c->u_in, sizeof(double) * c->Ny * 2); }
+
+/* FEMBodyClass::constructRVector, FEMBody.cpp:226-233: the load vector of
+ * flexible body b from its markers' present forces and epsilon. */
+int orc_fem_load(orc_case *c, int b, double *R) {
+    fem_t *F = c->b[b].fem;
+    if (!F) return -1;
+    memset(F->R, 0, sizeof(double) * F->ndof);
+    for (int el = 0; el < F->nel; el++) el_load(c, F, &F->el[el]);
+    memcpy(R, F->R, sizeof(double) * F->ndof);
+    return F->ndof;
 }

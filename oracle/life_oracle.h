@@ -90,6 +90,21 @@ void orc_equilibrium(int cm, double rho, double ux, double uy, double *feq);
 int orc_fem_cantilever(double L, double h, double E, double rho, int nel, double Dt, double P, int nsteps,
                        double *tip_out);
 
+/* Unit-level access, for checking each restated step on its own. */
+void orc_ibm_interpolate(orc_case *c);
+void orc_ibm_spread(orc_case *c);
+void orc_set_markers(orc_case *c, const double *pos, const double *vel, const double *force, const double *eps,
+                     const double *ds);
+int orc_ibm_supports(orc_case *c);
+int orc_epsilon_matrix(const orc_case *c, int sys, double *A);
+void orc_get_interp(const orc_case *c, double *rho, double *mom);
+int orc_normal_vector(const orc_case *c, int i, int j, int *n, int *nd);
+void orc_get_delu(const orc_case *c, double *delU);
+void orc_get_inlet(const orc_case *c, double *u_in);
+int orc_fem_load(orc_case *c, int b, double *R);
+double orc_aitken_relax(int subIt, double relax, double relaxMax, double num, double den);
+void orc_sub_residual(int nd, const double *U, const double *Ukm1, double *Rk, const double *Rkm1, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,19 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_fem_cantilever.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                                          C.c_double, C.c_int, dp]
+        L.orc_ibm_interpolate.argtypes = [vp]
+        L.orc_ibm_spread.argtypes = [vp]
+        L.orc_set_markers.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.orc_ibm_supports.argtypes = [vp]
+        L.orc_epsilon_matrix.argtypes = [vp, C.c_int, dp]
+        L.orc_get_interp.argtypes = [vp, dp, dp]
+        L.orc_normal_vector.argtypes = [vp, C.c_int, C.c_int, ip, ip]
+        L.orc_get_delu.argtypes = [vp, dp]
+        L.orc_get_inlet.argtypes = [vp, dp]
+        L.orc_fem_load.argtypes = [vp, C.c_int, dp]
+        L.orc_aitken_relax.restype = C.c_double
+        L.orc_aitken_relax.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_sub_residual.argtypes = [C.c_int, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -221,6 +234,64 @@ class OracleCase:
         lib().orc_total_force(self.h, _dp(out))
         return out
 
+    # -- unit-level access to single restated steps --------------------------
+
+    def ibm_interpolate(self):
+        lib().orc_ibm_interpolate(self.h)
+
+    def ibm_spread(self):
+        lib().orc_ibm_spread(self.h)
+
+    def set_markers(self, pos=None, vel=None, force=None, eps=None, ds=None):
+        keep = []
+
+        def arg(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            keep.append(a)
+            return _dp(a)
+        lib().orc_set_markers(self.h, arg(pos), arg(vel), arg(force), arg(eps), arg(ds))
+
+    def ibm_supports(self):
+        if lib().orc_ibm_supports(self.h) != 0:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+    def epsilon_matrix(self, sys=0):
+        n = lib().orc_epsilon_matrix(self.h, int(sys), None)
+        A = np.zeros((n, n))
+        lib().orc_epsilon_matrix(self.h, int(sys), _dp(A))
+        return A
+
+    def interp(self):
+        n = self.n_markers
+        rho, mom = np.zeros(n), np.zeros((n, 2))
+        lib().orc_get_interp(self.h, _dp(rho), _dp(mom))
+        return rho, mom
+
+    def normal(self, i, j):
+        n = np.zeros(2, dtype=np.int32)
+        nd = C.c_int(0)
+        if lib().orc_normal_vector(self.h, int(i), int(j), _ip(n), C.byref(nd)) != 0:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+        return (int(n[0]), int(n[1])), int(nd.value)
+
+    def delu(self):
+        d = np.zeros((self.Ny, 2))
+        lib().orc_get_delu(self.h, _dp(d))
+        return d
+
+    def inlet(self):
+        d = np.zeros((self.Ny, 2))
+        lib().orc_get_inlet(self.h, _dp(d))
+        return d
+
+    def fem_load(self, b):
+        nd = self.body_info(b)["ndof"]
+        R = np.zeros(nd)
+        lib().orc_fem_load(self.h, int(b), _dp(R))
+        return R
+
 
 def dirac_delta(x):
     return lib().orc_dirac_delta(float(x))
@@ -255,6 +326,21 @@ def fem_cantilever(L, h, E, rho, nel, Dt, P, nsteps=0):
     lib().orc_fem_cantilever(float(L), float(h), float(E), float(rho), int(nel), float(Dt), float(P),
                              int(nsteps), _dp(out))
     return out[0] if nsteps == 0 else out
+
+
+def aitken_relax(sub_it, relax, relax_max, num, den):
+    return lib().orc_aitken_relax(int(sub_it), float(relax), float(relax_max), float(num), float(den))
+
+
+def sub_residual(U, Ukm1, Rkm1):
+    """(R_k, |R_k|^2, numerator, denominator) of the subiteration residual."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Ukm1 = np.ascontiguousarray(Ukm1, dtype=np.float64)
+    Rkm1 = np.ascontiguousarray(Rkm1, dtype=np.float64)
+    Rk = np.zeros_like(U)
+    out = np.zeros(3)
+    lib().orc_sub_residual(len(U), _dp(U), _dp(Ukm1), _dp(Rk), _dp(Rkm1), _dp(out))
+    return Rk, out[0], out[1], out[2]
 
 
 def equilibrium(cm, rho, ux, uy):
