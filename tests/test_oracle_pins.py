@@ -382,6 +382,8 @@ def test_epsilon_matrix_brute_force(oracle, uni):
     pos[3] += [0.0, -0.95]  # a circle marker near the bottom wall
     c.set_markers(pos=pos)
     c.ibm_supports()
+    rng = np.random.default_rng(48)
+    c.set_markers(ds=rng.uniform(0.7, 1.3, c.n_markers))  # distinct ds: the factor must take the column's
     m = c.markers()
     systems = [np.arange(c.n_markers)] if uni else [
         np.arange(c.body_info(b)["first"], c.body_info(b)["first"] + c.body_info(b)["n"]) for b in range(c.n_bodies)]
