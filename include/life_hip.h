@@ -71,6 +71,13 @@ const char *life_last_error(void);
 const char *life_version(void);
 /* Number of visible HIP devices (0 without a GPU). */
 int life_device_count(int *count);
+/* Free the library's pooled HIP resources: idle streams and the device and
+ * page-locked blocks kept for reuse by later grids, and -- when no grid or
+ * server is live -- the shared CU-masked streams.  The last destroy of a
+ * grid or server does the same by itself; a host calls this before it
+ * exits with handles it never destroyed.  No reference counterpart (the
+ * reference allocates no device resources). */
+int life_release_cached(void);
 
 /* ---------------------------------------------------------------- layer 1 */
 
@@ -98,6 +105,10 @@ int life_grid_upload(life_grid *g, const double *f_aos, const double *rho, const
                      const double *force_ibm);
 /* Wait for all work queued on the grid's stream. */
 int life_grid_sync(life_grid *g);
+/* The NaN test of GridClass::writeInfo (Grid.cpp:562-581) on the current
+ * state: *i, *j = the first site in the reference's i-major order whose
+ * velocity magnitude is NaN, or -1, -1 when there is none.  Blocks. */
+int life_grid_blowup_check(life_grid *g, int *i, int *j);
 
 /* Markers: replaces the IBMNodeClass arrays (IBMNode.h:45-70).  n markers in
  * the reference's global order; body[k] is the index of marker k's body
@@ -148,6 +159,14 @@ int life_case_step(life_case *c, int nsteps);
  * launches alone (ms, summed). */
 int life_case_step_timed(life_case *c, int nsteps, double *ms_total, double *ms_kernel);
 int life_case_time(const life_case *c, int *t);
+/* Cadence of the blow-up test (the NaN velocity scan of GridClass::writeInfo,
+ * Grid.cpp:562-581, which the reference runs every tinfo steps): every
+ * `every` steps (default 100; 0 turns it off) life_case_step scans the
+ * populations on the device.  A NaN makes life_case_step return non-zero
+ * with "Simulation blew up (t = ..) at i = .., j = ..", t being the step
+ * scanned and (i, j) the first such site in i-major order; the steps after
+ * it that the call had already queued have run. */
+int life_case_set_check_interval(life_case *c, int every);
 life_grid *life_case_grid(life_case *c);
 int life_case_num_markers(const life_case *c, int *n);
 int life_case_num_bodies(const life_case *c, int *n);
@@ -203,6 +222,8 @@ typedef struct life_server_config {
     int max_cases;      /* admission slots */
     double hbm_budget;  /* bytes of device memory the admitted cases may hold */
     int use_graph;      /* 1: lattice phase as one hipGraph launch per step */
+    int check_every;    /* blow-up test cadence in steps (as life_case_set_check_interval);
+                           0: every 100 steps, < 0: off */
 } life_server_config;
 
 enum { LIFE_REQ_QUEUED = 0, LIFE_REQ_RUNNING = 1, LIFE_REQ_DONE = 2, LIFE_REQ_FAILED = 3 };

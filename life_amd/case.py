@@ -47,6 +47,12 @@ class Grid:
     def sync(self):
         L.check(L.load().life_grid_sync(self.h))
 
+    def blowup_check(self):
+        """(i, j) of the first site with a NaN velocity, or None."""
+        i, j = C.c_int(-1), C.c_int(-1)
+        L.check(L.load().life_grid_blowup_check(self.h, C.byref(i), C.byref(j)))
+        return None if i.value < 0 else (i.value, j.value)
+
     def download(self):
         f = np.zeros(self.N * 9)
         rho = np.zeros(self.N)
@@ -143,6 +149,10 @@ class Case:
 
     def step(self, n=1):
         L.check(L.load().life_case_step(self.h, int(n)))
+
+    def set_check_interval(self, every):
+        """Blow-up test cadence in steps (0: off); see life_case_set_check_interval."""
+        L.check(L.load().life_case_set_check_interval(self.h, int(every)))
 
     def step_timed(self, n):
         tot, ker = C.c_double(0), C.c_double(0)

@@ -398,6 +398,15 @@ void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, bool more, 
     T.t[5] += 1;
 }
 
+// The blow-up test of the reference's writeInfo (Grid.cpp:562-581) at the
+// grid's cadence, after step c->t.
+void case_blowup_cadence(life_case* c) {
+    life_grid* g = c->g;
+    if (g->check_every <= 0 || c->t % g->check_every != 0) return;
+    lbm_join(g);
+    blowup_cadence(g, c->t, g->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -477,11 +486,20 @@ int life_case_destroy(life_case* c) {
 int life_case_step(life_case* c, int nsteps) {
     return guarded([&] {
         LIFE_HIP(hipSetDevice(c->g->device));
-        for (int s = 0; s < nsteps; s++) case_step(c, nullptr, nullptr, s + 1 < nsteps, nullptr, nullptr);
+        for (int s = 0; s < nsteps; s++) {
+            case_step(c, nullptr, nullptr, s + 1 < nsteps, nullptr, nullptr);
+            case_blowup_cadence(c);
+        }
         lbm_join(c->g);
         LIFE_HIP(hipStreamSynchronize(c->g->stream));
+        blowup_check(c->g, true);
         if (c->hasIBM) check_flags(c->g);
     });
+}
+
+int life_case_set_check_interval(life_case* c, int every) {
+    c->g->check_every = every > 0 ? every : 0;
+    return 0;
 }
 
 int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_kernel) {
@@ -508,10 +526,12 @@ int life_case_step_timed(life_case* c, int nsteps, double* ms_total, double* ms_
             hipEvent_t ne = s + 1 < nsteps ? c->kev[2 * (s + 1) + 1] : nullptr;
             case_step(c, early ? nullptr : c->kev[2 * s], early ? nullptr : c->kev[2 * s + 1], s + 1 < nsteps, nb,
                       ne);
+            case_blowup_cadence(c);
         }
         lbm_join(g);
         LIFE_HIP(hipEventRecord(g->ev_b, g->stream));
         LIFE_HIP(hipEventSynchronize(g->ev_b));
+        blowup_check(g, true);
         float ms = 0.0f;
         LIFE_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
         double kms = 0.0;

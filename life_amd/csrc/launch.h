@@ -48,6 +48,10 @@ void launch_upload(T* f, const GridScalars& g, const double* f_aos, hipStream_t 
 void launch_init_moments(double* rho0, double* u0, const GridScalars& g, int mode, double ux, double uy,
                          const double* u_in, const BCSite* bc, int nbc, hipStream_t s);
 void launch_fibm_from_aos(double* fibm, const GridScalars& g, const double* fibm_aos, hipStream_t s);
+// NaN velocity scan (GridClass::writeInfo): *flag = smallest lattice id
+// whose |u| is NaN, or ~0 when there is none.
+template <typename T>
+void launch_blowup_scan(const T* f, const GridScalars& g, unsigned long long* flag, hipStream_t s);
 void launch_fill_f64(double* p, double v, long long n, hipStream_t s);
 void launch_fill_i32(int* p, int v, long long n, hipStream_t s);
 

@@ -448,7 +448,41 @@ void launch_init_moments(double* rho0, double* u0, const GridScalars& g, int mod
     if (nbc > 0) k_zero_wall_u<<<(nbc + 255) / 256, 256, 0, s>>>(u0, bc, nbc);
 }
 
+// The blow-up test of GridClass::writeInfo (Grid.cpp:562-581): the
+// velocity magnitude of every site from its populations, and the smallest
+// lattice id (i-major, the reference's scan order) whose magnitude is NaN
+// into *flag (left untouched when there is none).  u = j / rho without the
+// half-force terms: a finite force changes no NaN.
+template <typename T>
+__global__ __launch_bounds__(256) void k_blowup_scan(const T* __restrict__ f, long long S, long long N,
+                                                     unsigned long long* __restrict__ flag) {
+    for (long long id = (long long)blockIdx.x * 256 + threadIdx.x; id < N; id += (long long)gridDim.x * 256) {
+        double fs[kNV];
+        load_site(f, S, id, fs);
+        double r = 0.0, jx = 0.0, jy = 0.0;
+#pragma unroll
+        for (int v = 0; v < kNV; v++) {
+            r += fs[v];
+            jx += kCX[v] * fs[v];
+            jy += kCY[v] * fs[v];
+        }
+        const double ux = jx / r, uy = jy / r;
+        if (__builtin_isnan(sqrt(ux * ux + uy * uy))) atomicMin(flag, (unsigned long long)id);
+    }
+}
+
+__global__ void k_flag_reset(unsigned long long* flag) { *flag = ~0ull; }
+
+template <typename T>
+void launch_blowup_scan(const T* f, const GridScalars& g, unsigned long long* flag, hipStream_t s) {
+    k_flag_reset<<<1, 1, 0, s>>>(flag);
+    long long nb = (g.N + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    k_blowup_scan<T><<<(unsigned)nb, 256, 0, s>>>(f, g.S, g.N, flag);
+}
+
 #define LIFE_INST(T)                                                                                                 \
+    template void launch_blowup_scan<T>(const T*, const GridScalars&, unsigned long long*, hipStream_t);             \
     template void launch_bc<T>(const T*, T*, const GridScalars&, const BCArgs&, const double*, const double*,        \
                                const double*, bool, hipStream_t);                                                    \
     template void launch_convective_speed<T>(const T*, const GridScalars&, const double*, const double*, bool,       \

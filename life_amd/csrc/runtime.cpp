@@ -49,6 +49,16 @@ void* pinned_take(size_t bytes) {
     return p;
 }
 
+// Free every kept pinned block.
+void pinned_release_all() {
+    PinnedCache& pc = pinned_cache();
+    std::lock_guard<std::mutex> lock(pc.mu);
+    for (auto& kv : pc.blocks)
+        for (void* p : kv.second) (void)hipHostFree(p);
+    pc.blocks.clear();
+    pc.bytes = 0;
+}
+
 void pinned_give(void* p, size_t bytes) {
     if (!p) return;
     PinnedCache& pc = pinned_cache();
@@ -368,6 +378,48 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
     g->explicit_state = false;
 }
 
+void blowup_scan(life_grid* g, int t, hipStream_t s) {
+    if (!g->blow_d) {
+        g->blow_d = dalloc<unsigned long long>(1);
+        LIFE_HIP(hipHostMalloc((void**)&g->blow_h, sizeof(unsigned long long), hipHostMallocDefault));
+        LIFE_HIP(hipEventCreateWithFlags(&g->ev_blow, hipEventDisableTiming));
+    }
+    if (g->prec == kFP64) launch_blowup_scan<double>(static_cast<const double*>(g->f[g->cur]), g->g, g->blow_d, s);
+    else launch_blowup_scan<float>(static_cast<const float*>(g->f[g->cur]), g->g, g->blow_d, s);
+    LIFE_HIP(hipGetLastError());
+    LIFE_HIP(hipMemcpyAsync(g->blow_h, g->blow_d, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    LIFE_HIP(hipEventRecord(g->ev_blow, s));
+    g->blow_t = t;
+}
+
+void blowup_check(life_grid* g, bool wait) {
+    if (g->blow_t < 0) return;
+    if (!wait) {
+        const hipError_t q = hipEventQuery(g->ev_blow);
+        if (q == hipErrorNotReady) return;
+        LIFE_HIP(q);
+    } else {
+        LIFE_HIP(hipEventSynchronize(g->ev_blow));
+    }
+    const int t = g->blow_t;
+    g->blow_t = -1;
+    const unsigned long long id = *g->blow_h;
+    if (id == ~0ull) return;
+    const long long Ny = g->g.Ny;
+    // GridClass::writeInfo, Grid.cpp:573-579
+    throw std::runtime_error("Simulation blew up (t = " + std::to_string(t) + ") at i = " +
+                             std::to_string((long long)id / Ny) + ", j = " + std::to_string((long long)id % Ny) +
+                             "...exiting");
+}
+
+void blowup_cadence(life_grid* g, int t, hipStream_t s) {
+    if (g->check_every <= 0 || t % g->check_every != 0) return;
+    // the previous scan was queued check_every steps ago: usually done, else
+    // the host waits for it (so it never runs more than that far ahead)
+    blowup_check(g, true);
+    blowup_scan(g, t, s);
+}
+
 void ibm_interpolate(life_grid* g) {
     if (g->md.n <= 0) return;
     lbm_join(g);
@@ -476,23 +528,41 @@ static void free_markers(life_grid* g) {
     g->flags = nullptr;
 }
 
+namespace {
+// CU-masked streams, one per (device, reserve), shared by every grid and
+// server on the device while any handle is live.
+struct MaskedStreams {
+    std::mutex mu;
+    std::map<std::pair<int, int>, hipStream_t> streams;
+};
+MaskedStreams& masked_streams() {
+    static MaskedStreams* m = new MaskedStreams();
+    return *m;
+}
+
+// Live grids and servers.  When the last one goes, every pooled HIP
+// resource is released (release_cached), so that nothing the library owns
+// is left for the HIP runtime's own teardown at process exit.
+std::mutex g_handles_mu;
+int g_handles = 0;
+}  // namespace
+
 // The stream is masked to all CUs but the reserve.  The reserved CUs are
 // spread over the 8 XCDs whether the mask bits number the CUs XCD by XCD or
 // round-robin over the XCDs: CU x*G + ((x + 8*k) mod G) for XCD x and
 // k = 0, 1, ... (G = CUs per XCD) falls on XCD x either way.  A CU-masked
 // stream has a hardware queue of its own, so there is one per device and
-// reserve for the whole process, shared by every grid or server on it
-// (never destroyed).  Sharing cannot deadlock: a user records the event
-// its kernel waits for before it queues that wait.
+// reserve, shared by every grid or server on it and destroyed when the
+// last of them is.  Sharing cannot deadlock: a user records the event its
+// kernel waits for before it queues that wait.
 hipStream_t masked_stream(int device, int reserve_cus) {
-    static std::mutex mu;
-    static std::map<std::pair<int, int>, hipStream_t> streams;
-    std::lock_guard<std::mutex> lock(mu);
+    MaskedStreams& ms = masked_streams();
+    std::lock_guard<std::mutex> lock(ms.mu);
     int ncu = 0;
     LIFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     const int reserve = reserve_cus < 0 ? 0 : (reserve_cus < ncu / 8 ? reserve_cus : ncu / 8);  // at most G/8 per XCD
-    auto it = streams.find({device, reserve});
-    if (it != streams.end()) return it->second;
+    auto it = ms.streams.find({device, reserve});
+    if (it != ms.streams.end()) return it->second;
     hipStream_t st = nullptr;
     if (reserve == 0 || ncu % 8 != 0) {
         LIFE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -507,8 +577,48 @@ hipStream_t masked_stream(int device, int reserve_cus) {
         }
         LIFE_HIP(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
     }
-    streams[{device, reserve}] = st;
+    ms.streams[{device, reserve}] = st;
     return st;
+}
+
+// Frees the pools: idle grid streams, kept device and pinned blocks, and,
+// with shared_streams, the CU-masked streams (only when no grid or server
+// is live: the caller holds g_handles_mu and checked).
+static void release_pools(bool shared_streams) {
+    {
+        std::lock_guard<std::mutex> lock(g_stream_mu);
+        for (auto& kv : g_free_streams)
+            for (hipStream_t st : kv.second) (void)hipStreamDestroy(st);
+        g_free_streams.clear();
+    }
+    {
+        DeviceCache& dc = device_cache();
+        std::lock_guard<std::mutex> lock(dc.mu);
+        for (auto& kv : dc.blocks)
+            for (void* p : kv.second) (void)hipFree(p);
+        dc.blocks.clear();
+        dc.bytes = 0;
+    }
+    pinned_release_all();
+    if (shared_streams) {
+        MaskedStreams& ms = masked_streams();
+        std::lock_guard<std::mutex> lock(ms.mu);
+        for (auto& kv : ms.streams) {
+            (void)hipStreamSynchronize(kv.second);
+            (void)hipStreamDestroy(kv.second);
+        }
+        ms.streams.clear();
+    }
+}
+
+void handle_acquire() {
+    std::lock_guard<std::mutex> lock(g_handles_mu);
+    ++g_handles;
+}
+
+void handle_release() {
+    std::lock_guard<std::mutex> lock(g_handles_mu);
+    if (--g_handles == 0) release_pools(true);
 }
 
 }  // namespace life
@@ -520,6 +630,13 @@ extern "C" {
 const char* life_last_error(void) { return life::g_error.c_str(); }
 
 const char* life_version(void) { return "life_hip 0.1 (gfx950)"; }
+
+int life_release_cached(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_handles_mu);
+        release_pools(g_handles == 0);
+    });
+}
 
 int life_device_count(int* count) {
     return guarded([&] {
@@ -567,6 +684,7 @@ hipStream_t bulk_stream(int device) { return masked_stream(device, coupling_cus(
 int life_grid_create(const life_params* p, int precision, int device, life_grid** out) {
     *out = nullptr;
     life_grid* g = new life_grid();
+    handle_acquire();
     int rc = guarded([&] {
         if (p->Nx < 4 || p->Ny < 4) throw std::runtime_error("grid must be at least 4x4");
         if (precision != LIFE_FP64 && precision != LIFE_FP32_SHIFTED) throw std::runtime_error("unknown precision");
@@ -708,7 +826,7 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
 
 int life_grid_destroy(life_grid* g) {
     if (!g) return 0;
-    return guarded([&] {
+    const int rc = guarded([&] {
         (void)hipSetDevice(g->device);
         if (g->stream) (void)hipStreamSynchronize(g->stream);
         if (g->ev_bulk) (void)hipEventSynchronize(g->ev_bulk);  // this grid's last bulk kernel
@@ -721,11 +839,16 @@ int life_grid_destroy(life_grid* g) {
                         (void*)g->eps_band, (void*)g->eps_y})
             dfree(p);
         if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
+        if (g->blow_h) (void)hipHostFree(g->blow_h);
+        dfree(g->blow_d);
+        if (g->ev_blow) (void)hipEventDestroy(g->ev_blow);
         for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk, g->ev_tail})
             if (e) (void)hipEventDestroy(e);
         if (g->stream) give_grid_stream(g->device, g->stream);
         delete g;
     });
+    handle_release();
+    return rc;
 }
 
 int life_grid_scalars(const life_grid* g, double out[10]) {
@@ -740,6 +863,23 @@ int life_lbm_kernel(life_grid* g, int t) {
         LIFE_HIP(hipSetDevice(g->device));
         lbm_join(g);
         lbm_step(g, t, nullptr, nullptr);
+    });
+}
+
+int life_grid_blowup_check(life_grid* g, int* i, int* j) {
+    return guarded([&] {
+        LIFE_HIP(hipSetDevice(g->device));
+        lbm_join(g);
+        *i = *j = -1;
+        blowup_check(g, true);  // a scan still in flight from life_case_step is read first
+        blowup_scan(g, g->t_step, g->stream);
+        LIFE_HIP(hipEventSynchronize(g->ev_blow));
+        g->blow_t = -1;
+        const unsigned long long id = *g->blow_h;
+        if (id != ~0ull) {
+            *i = (int)((long long)id / g->g.Ny);
+            *j = (int)((long long)id % g->g.Ny);
+        }
     });
 }
 

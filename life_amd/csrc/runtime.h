@@ -112,6 +112,15 @@ struct life_grid {
     bool bulk_pending = false;  // a bulk kernel of the next step is queued on stream2
     bool tail_pending = false;  // the last step's tail ran on stream2 (stream not yet joined)
     int t_step = 0;             // time step of the last lbm_step
+    // blow-up scan (GridClass::writeInfo's NaN test): every check_every
+    // steps (0: never) the populations are scanned on the device; the
+    // result reaches blow_h asynchronously and is read at the next scan or
+    // at the end of the call
+    int check_every = 100;
+    unsigned long long* blow_d = nullptr;
+    unsigned long long* blow_h = nullptr;  // page-locked
+    hipEvent_t ev_blow = nullptr;
+    int blow_t = -1;  // step of the scan in flight, -1 none
 
     void* f_cur() { return f[cur]; }
 };
@@ -140,12 +149,25 @@ void lbm_join(life_grid* g);
 // One per (device, reserve) for the whole process, never destroyed: a
 // CU-masked stream holds a hardware queue of its own.
 hipStream_t masked_stream(int device, int reserve_cus);
+// Count of live grids and servers; the last release frees every pooled HIP
+// resource (idle streams, kept device/pinned blocks, CU-masked streams).
+void handle_acquire();
+void handle_release();
 // force_xy of time step t (uniform): the initial value (Grid.cpp:1042-1043),
 // or under WOMERSLEY the oscillating pressure gradient of step t
 // (Grid.cpp:52-61).
 void body_force(const life_grid& g, int t, double F[2]);
 // The grid's kernel scalars for time step t: force_xy of t and of t - 1.
 GridScalars step_scalars(const life_grid& g, int t);
+// Queue a blow-up scan of the current populations (step t) on stream s.
+void blowup_scan(life_grid* g, int t, hipStream_t s);
+// Read the scan in flight: with wait, after it completes; without, only if
+// it already has.  Throws "Simulation blew up (t = .) at i = ., j = ." when
+// it found a NaN velocity.
+void blowup_check(life_grid* g, bool wait);
+// Scan after step t when the cadence says so (reading the previous scan
+// first if it is done).
+void blowup_cadence(life_grid* g, int t, hipStream_t s);
 void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);

@@ -246,6 +246,7 @@ void admit(life_server* s) {
         // force_ibm is zero from creation and no spread has run, so the
         // first spread needs no full clear
         c->g->dense_clear_next = false;
+        c->g->check_every = s->cfg.check_every == 0 ? 100 : (s->cfg.check_every > 0 ? s->cfg.check_every : 0);
         r->info.status = LIFE_REQ_RUNNING;
         r->info.markers = c->nm;
         s->slots.push_back(Slot{r, c, 0});
@@ -687,10 +688,26 @@ void engine_step(life_server* s) {
     }
     if (s->early == 1) launch_bulk_early(s);
     coupling_phase(s);
+    // the blow-up test of the cases due (GridClass::writeInfo's NaN scan
+    // at the tinfo cadence); it only reads the populations, so a case's
+    // neighbours are unaffected whatever it finds
+    for (Slot& sl : s->slots) {
+        life_grid* g = sl.c->g;
+        if (g->check_every > 0 && sl.c->t % g->check_every == 0 && sl.req->info.status != LIFE_REQ_FAILED)
+            blowup_scan(g, sl.c->t, s->stream);
+    }
     // no FEM this step (or it failed): the next bulk kernel still runs
     // beside the spread and the step's end
     if (s->early == 2) launch_bulk_early(s);
     LIFE_HIP(hipStreamSynchronize(s->stream));
+    for (Slot& sl : s->slots) {
+        if (sl.c->g->blow_t < 0) continue;
+        try {
+            blowup_check(sl.c->g, true);
+        } catch (const std::exception& e) {
+            fail(s, sl, e.what());  // retired below, its slot and memory freed
+        }
+    }
     // a case that retires below may still be read by the early kernel, and
     // cases_bulk's host copy is written again next step
     LIFE_HIP(hipStreamSynchronize(s->stream2));
@@ -745,6 +762,7 @@ extern "C" {
 int life_server_create(const life_server_config* cfg, life_server** out) {
     *out = nullptr;
     life_server* s = new life_server();
+    handle_acquire();
     const int rc = guarded([&] {
         if (!cfg) throw std::runtime_error("null config");
         if (cfg->precision != LIFE_FP64 && cfg->precision != LIFE_FP32_SHIFTED)
@@ -796,6 +814,7 @@ int life_server_destroy(life_server* s) {
         if (s->stream) (void)hipStreamDestroy(s->stream);
     });
     delete s;
+    handle_release();
     return rc;
 }
 

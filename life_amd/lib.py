@@ -3,6 +3,7 @@
 Loading fails loudly when the library has not been built: the package has
 no CPU fallback for any entry point.
 """
+import atexit
 import ctypes as C
 import os
 
@@ -46,7 +47,7 @@ def make_params(p):
 class ServerConfig(C.Structure):
     """struct life_server_config (include/life_hip.h)."""
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("max_cases", C.c_int),
-                ("hbm_budget", C.c_double), ("use_graph", C.c_int)]
+                ("hbm_budget", C.c_double), ("use_graph", C.c_int), ("check_every", C.c_int)]
 
 
 LIFE_REQ_QUEUED, LIFE_REQ_RUNNING, LIFE_REQ_DONE, LIFE_REQ_FAILED = 0, 1, 2, 3
@@ -68,6 +69,7 @@ SIGNATURES = {
     "life_last_error": (C.c_char_p, []),
     "life_version": (C.c_char_p, []),
     "life_device_count": (C.c_int, [_ip]),
+    "life_release_cached": (C.c_int, []),
     "life_grid_create": (C.c_int, [C.POINTER(LifeParams), C.c_int, C.c_int, C.POINTER(_vp)]),
     "life_grid_destroy": (C.c_int, [_vp]),
     "life_grid_scalars": (C.c_int, [_vp, _dp]),
@@ -75,6 +77,7 @@ SIGNATURES = {
     "life_grid_download": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "life_grid_upload": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "life_grid_sync": (C.c_int, [_vp]),
+    "life_grid_blowup_check": (C.c_int, [_vp, _ip, _ip]),
     "life_ibm_set_markers": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, C.c_int, _ip]),
     "life_ibm_update_markers": (C.c_int, [_vp, _dp, _dp]),
     "life_ibm_find_support": (C.c_int, [_vp, C.c_int]),
@@ -89,6 +92,7 @@ SIGNATURES = {
     "life_case_step": (C.c_int, [_vp, C.c_int]),
     "life_case_step_timed": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "life_case_time": (C.c_int, [_vp, _ip]),
+    "life_case_set_check_interval": (C.c_int, [_vp, C.c_int]),
     "life_case_grid": (_vp, [_vp]),
     "life_case_num_markers": (C.c_int, [_vp, _ip]),
     "life_case_num_bodies": (C.c_int, [_vp, _ip]),
@@ -132,6 +136,9 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        # pooled streams and blocks go back to HIP before the interpreter
+        # (and with it the HIP runtime) shuts down
+        atexit.register(L.life_release_cached)
     return _lib
 
 

@@ -9,9 +9,10 @@ from . import lib as L
 
 
 class Server:
-    def __init__(self, precision=L.LIFE_FP32_SHIFTED, max_cases=64, hbm_budget=200e9, use_graph=True, device=0):
+    def __init__(self, precision=L.LIFE_FP32_SHIFTED, max_cases=64, hbm_budget=200e9, use_graph=True, device=0,
+                 check_every=0):
         self.cfg = L.ServerConfig(int(device), int(precision), int(max_cases), float(hbm_budget),
-                                  1 if use_graph else 0)
+                                  1 if use_graph else 0, int(check_every))
         h = C.c_void_p()
         L.check(L.load().life_server_create(C.byref(self.cfg), C.byref(h)))
         self.h = h
