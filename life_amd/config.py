@@ -126,6 +126,66 @@ def pelskin(resFactor=3):
     return p, geom
 
 
+def reference_example(name):
+    """The reference's own example cases (examples/<name>/params.h and
+    input/geometry.config, run for 500 steps by testing/run-tests.sh) in
+    their own physical units; omega follows from tStep as params.h:89-90
+    derives it.  PELskin is :func:`pelskin`."""
+    if name == "ChannelFlow":
+        # BGK, uniform inlet, pressure outlet, no bodies
+        p = CaseParams(Nx=500 + 1, Ny=50 + 1, height_p=1.0, rho_p=1.0, nu_p=0.01, wall_left=VELOCITY,
+                       wall_right=PRESSURE, wall_bottom=WALL, wall_top=WALL, profile=UNIFORM, uxInlet_p=1.0,
+                       tStep=0.0005, ref_nu=0.01, ref_rho=1.0, ref_L=1.0, ref_U=1.0)
+        return p, ""
+    if name == "Cylinder":
+        # Schaefer-Turek channel with a rigid cylinder, BGK, parabolic inlet
+        # ramped over 2 s
+        r = 2
+        p = CaseParams(Nx=r * 220 + 1, Ny=r * 41 + 1, height_p=0.41, rho_p=1.0, nu_p=0.001, wall_left=VELOCITY,
+                       wall_right=PRESSURE, wall_bottom=WALL, wall_top=WALL, profile=PARABOLIC, uxInlet_p=1.0,
+                       tStep=0.001 / (r * r), inlet_ramp=2.0, ref_nu=0.001, ref_rho=1.0, ref_L=0.1, ref_U=1.0)
+        return p, "CIRCLE 1 0 0.2 0.2 0.0 0.0 0.05\n"
+    if name == "Honami":
+        # 128 clamped flexible filaments on the bottom wall, boundary-layer
+        # inlet ramped over 250 s, free-slip top
+        p = CaseParams(Nx=1200 + 127 * 15 + 1, Ny=90 + 1, height_p=3.0, rho_p=1.0, nu_p=0.0125, wall_left=VELOCITY,
+                       wall_right=PRESSURE, wall_bottom=WALL, wall_top=FREESLIP, profile=BOUNDARY_LAYER,
+                       uxInlet_p=1.0, tStep=0.00625 / (3.0 * 3.0), inlet_ramp=250.0, ref_nu=0.0125, ref_rho=1.0,
+                       ref_L=1.0, ref_U=1.0)
+        return p, "FILAMENT 128 0 10.0 0.0 0.5 0.0 1.0 0.02 90.0 FLEXIBLE 20 CLAMPED 50.000000 269536.024785\n"
+    if name == "InvertedFlag":
+        # one plate clamped at its trailing edge with an initial deflection,
+        # uniform flow, convective outlet
+        r = 3
+        p = CaseParams(Nx=r * 10 * 21 + 1, Ny=r * 10 * 15 + 1, height_p=0.855, rho_p=1.2047, nu_p=1.5111e-2,
+                       ux0_p=8.0, wall_left=VELOCITY, wall_right=CONVECTIVE, wall_bottom=VELOCITY,
+                       wall_top=VELOCITY, profile=UNIFORM, uxInlet_p=8.0, subTol=1e-5, tStep=0.00005 / (r * r),
+                       initial_deflect=0.01, ref_nu=1.5111e-2, ref_rho=1.2047, ref_L=0.057, ref_U=8.0)
+        return p, "FILAMENT 1 0 0.342 0.4275 0.0 0.0 0.057 0.000205 180.0 FLEXIBLE 20 CLAMPED 1780.0 3.0e9\n"
+    if name == "LidDrivenCavity":
+        # central moments, moving top wall
+        p = CaseParams(Nx=100 + 1, Ny=100 + 1, height_p=1.0, rho_p=1.0, nu_p=0.0002, wall_left=WALL,
+                       wall_right=WALL, wall_bottom=WALL, wall_top=VELOCITY, profile=UNIFORM, uxInlet_p=1.0,
+                       tStep=0.001, central_moments=1, ref_nu=0.0002, ref_rho=1.0, ref_L=1.0, ref_U=1.0)
+        return p, ""
+    if name == "TurekHron":
+        # FSI2-type benchmark: rigid cylinder + flexible plate, one epsilon
+        # system over all markers, parabolic inlet ramped over 2 s
+        r = 2
+        p = CaseParams(Nx=r * 250 + 1, Ny=r * 41 + 1, height_p=0.41, rho_p=1000.0, nu_p=0.001, wall_left=VELOCITY,
+                       wall_right=PRESSURE, wall_bottom=WALL, wall_top=WALL, profile=PARABOLIC, uxInlet_p=1.0,
+                       tStep=0.001 / (r * r), inlet_ramp=2.0, uni_epsilon=1, ref_nu=0.001, ref_rho=1000.0,
+                       ref_L=0.1, ref_U=1.0)
+        return p, ("CIRCLE 1 0 0.2 0.2 0.0 0.0 0.05\n"
+                   "FILAMENT 1 1 0.25 0.2 0.0 0.0 0.35 0.02 0.0 FLEXIBLE 20 CLAMPED 10000.0 1.4e6\n")
+    if name == "PELskin":
+        return pelskin()
+    raise KeyError(name)
+
+
+REFERENCE_EXAMPLES = ("ChannelFlow", "Cylinder", "Honami", "InvertedFlag", "LidDrivenCavity", "PELskin", "TurekHron")
+
+
 def womersley_profile(y, t, H, nu, Wo, G):
     """Planar Womersley flow between walls at y = 0 and y = H driven by the
     body force G cos(omega t): u = Re{G / (i omega) (1 - cosh(k (y - H/2)) /
