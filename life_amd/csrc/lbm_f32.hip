@@ -226,6 +226,110 @@ __global__ __launch_bounds__(256) void k_push32x4(const float* __restrict__ src,
     }
 }
 
+// Four sites per lane with LDS realignment, for Ny % 4 == 0 (a lane's four
+// sites then lie in one row, and every float4 of a plane is 16-byte
+// aligned).  Each population plane moves with one 16-byte load per lane;
+// the c_y = 0 populations go out with one 16-byte store (their
+// destination is the same four sites one row over).  The six c_y = +-1
+// populations land one site up or down the row: a workgroup drops them
+// into LDS at their destination offset within its block of 4*BS sites and
+// every lane then stores the float4 that lands on its own four sites.
+// Values that leave the block or wrap around the row (Grid.cpp:229, 240)
+// are stored directly by their source lane; a lane whose float4 holds a
+// destination with no source in the block (the block's first site for
+// c_y = +1, its last for c_y = -1) or a wrapped row end (j = 0 for
+// c_y = +1, j = Ny - 1 for c_y = -1) stores its other three elements one
+// by one.  The arithmetic is k_push32's, site by site.
+template <int BS>
+struct PushStage4 {
+    // 4 floats of padding at each end keep the lane reads 16-byte aligned
+    float slot[6][4 * BS + 8];
+};
+
+template <bool CM, int BS>
+__global__ __launch_bounds__(BS) void k_push32x4_lds(const float* __restrict__ src, float* __restrict__ dst,
+                                                     GridScalars g, unsigned nblocks) {
+    __shared__ __attribute__((aligned(16))) PushStage4<BS> st;
+    const int t = threadIdx.x;
+    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * (4 * BS);
+    const long long id0 = base + 4 * t;
+    const bool valid = id0 < g.N;  // N is a multiple of 4 (Ny is)
+    const long long S = g.S;
+    const int Ny = g.Ny, Nx = g.Nx;
+    const long long idc = valid ? id0 : 0;
+    const int i = (int)(idc / Ny), j0 = (int)(idc - (long long)i * Ny);
+    v4f in[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; v++)
+        in[v] = valid ? __builtin_nontemporal_load(reinterpret_cast<const v4f*>(src + v * S + idc))
+                      : v4f{0.f, 0.f, 0.f, 0.f};
+    float o[kNV][4];
+    const float Fx = (float)g.Fx, Fy = (float)g.Fy;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        float gs[kNV], go[kNV];
+#pragma unroll
+        for (int v = 0; v < kNV; v++) gs[v] = in[v][s];
+        float drho, ux, uy;
+        moments32(gs, (float)g.Fxm, (float)g.Fym, drho, ux, uy);
+        collide32<CM>(g, gs, drho, ux, uy, Fx, Fy, go);
+#pragma unroll
+        for (int v = 0; v < kNV; v++) o[v][s] = go[v];
+    }
+    const int im = i == 0 ? Nx - 1 : i - 1, ip = i == Nx - 1 ? 0 : i + 1;
+    const long long r0 = (long long)i * Ny, rp = (long long)ip * Ny, rm = (long long)im * Ny;
+    if (valid) {
+        __builtin_nontemporal_store(v4f{o[0][0], o[0][1], o[0][2], o[0][3]},
+                                    reinterpret_cast<v4f*>(dst + 0 * S + r0 + j0));
+        __builtin_nontemporal_store(v4f{o[1][0], o[1][1], o[1][2], o[1][3]},
+                                    reinterpret_cast<v4f*>(dst + 1 * S + rp + j0));
+        __builtin_nontemporal_store(v4f{o[2][0], o[2][1], o[2][2], o[2][3]},
+                                    reinterpret_cast<v4f*>(dst + 2 * S + rm + j0));
+    }
+    // cy = +1: v = 3 (cx 0), 5 (+1), 8 (-1); cy = -1: v = 4 (0), 6 (-1), 7 (+1)
+    constexpr int kv[6] = {3, 5, 8, 4, 6, 7};
+    constexpr int kcy[6] = {1, 1, 1, -1, -1, -1};
+    const long long rowv[6] = {r0, rp, rm, r0, rm, rp};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int cy = kcy[q];
+        if (!valid) continue;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int jd = j0 + s + cy;
+            const int to = 4 * t + s + cy;  // destination offset within the block
+            if (jd < 0 || jd >= Ny) dst[kv[q] * S + rowv[q] + (jd < 0 ? Ny - 1 : 0)] = o[kv[q]][s];
+            else if (to < 0 || to >= 4 * BS) dst[kv[q] * S + rowv[q] + jd] = o[kv[q]][s];
+            else st.slot[q][to + 4] = o[kv[q]][s];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    // lane t stores the values destined for sites 4t .. 4t+3 of the block;
+    // the one (at most) without a source in the block is left out
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int cy = kcy[q];
+        const v4f w = *reinterpret_cast<const v4f*>(&st.slot[q][4 * t + 4]);
+        float* row = dst + kv[q] * S + rowv[q];
+        // element without a source: first site of the block or a row start
+        // (cy = +1); last site of the block (or of the lattice) or a row end
+        // (cy = -1)
+        const bool gap = cy > 0 ? (t == 0 || j0 == 0) : (t == BS - 1 || id0 + 4 >= g.N || j0 + 4 == Ny);
+        if (!gap) {
+            __builtin_nontemporal_store(w, reinterpret_cast<v4f*>(row + j0));
+        } else if (cy > 0) {
+            row[j0 + 1] = w[1];
+            row[j0 + 2] = w[2];
+            row[j0 + 3] = w[3];
+        } else {
+            row[j0] = w[0];
+            row[j0 + 1] = w[1];
+            row[j0 + 2] = w[2];
+        }
+    }
+}
+
 template <bool CM>
 __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict__ src, float* __restrict__ dst,
                                                         GridScalars g, const double* __restrict__ fibm,
@@ -237,14 +341,18 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
 }
 
 // Launch configuration of the bulk stream-collide kernel, read once from
-// the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 | lds
-// (kernel), LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32),
+// the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 | lds |
+// x4lds (kernel; x4lds, the default, needs Ny % 4 == 0 and falls back to lds
+// otherwise), LIFE_PUSH32_X4BS = 64 | 128 | 256 | 512 (lanes per block of
+// k_push32x4_lds), LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32),
 // LIFE_PUSH32_XCD = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256
 // (block size of k_push32), LIFE_PUSH32_LDSBS = 256 | 512 | 1024 (block size
 // of k_push32_lds; 512 measured best on MI355X), LIFE_PUSH32_LDSNT = 0 | 1
 // (streaming stores in k_push32_lds).
 struct PushConfig {
     bool x4 = false;
+    bool x4lds = true;
+    int x4lds_bs = 256;
     bool lds = true;
     int nt = 3;
     bool xcd = true;
@@ -254,7 +362,12 @@ struct PushConfig {
     PushConfig() {
         if (const char* e = std::getenv("LIFE_PUSH32")) {
             x4 = std::strcmp(e, "x4") == 0;
-            lds = std::strcmp(e, "lds") == 0;
+            x4lds = std::strcmp(e, "x4lds") == 0;
+            lds = x4lds || std::strcmp(e, "lds") == 0;  // x4lds falls back to lds
+        }
+        if (const char* e = std::getenv("LIFE_PUSH32_X4BS")) {
+            const int v = std::atoi(e);
+            x4lds_bs = v == 64 || v == 128 || v == 512 ? v : 256;
         }
         if (const char* e = std::getenv("LIFE_PUSH32_NT")) nt = std::atoi(e) & 7;
         if (const char* e = std::getenv("LIFE_PUSH32_XCD")) xcd = std::atoi(e) != 0;
@@ -268,10 +381,8 @@ struct PushConfig {
     }
 };
 
-static const PushConfig& push_config() {
-    static const PushConfig c;
-    return c;
-}
+// Read at every launch, so that a test can switch kernels between steps.
+static PushConfig push_config() { return PushConfig(); }
 
 template <bool CM, int NT>
 static void launch_push32_nt(const float* src, float* dst, const GridScalars& g, const PushConfig& c, hipStream_t s) {
@@ -282,7 +393,18 @@ static void launch_push32_nt(const float* src, float* dst, const GridScalars& g,
 
 template <bool CM>
 static void launch_push32_bulk(const float* src, float* dst, const GridScalars& g, hipStream_t s) {
-    const PushConfig& c = push_config();
+    const PushConfig c = push_config();
+    if (c.x4lds && g.Ny % 4 == 0) {
+        const long long sites = 4LL * c.x4lds_bs;
+        const unsigned nb = (unsigned)((g.N + sites - 1) / sites);
+        switch (c.x4lds_bs) {
+            case 64: k_push32x4_lds<CM, 64><<<nb, 64, 0, s>>>(src, dst, g, nb); break;
+            case 128: k_push32x4_lds<CM, 128><<<nb, 128, 0, s>>>(src, dst, g, nb); break;
+            case 512: k_push32x4_lds<CM, 512><<<nb, 512, 0, s>>>(src, dst, g, nb); break;
+            default: k_push32x4_lds<CM, 256><<<nb, 256, 0, s>>>(src, dst, g, nb); break;
+        }
+        return;
+    }
     if (c.lds) {
         const unsigned nb = (unsigned)((g.N + c.lds_bs - 1) / c.lds_bs);
         if (c.lds_bs == 512) {

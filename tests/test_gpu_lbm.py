@@ -143,3 +143,26 @@ def test_fp32_shifted_long_horizon(oracle, gpu_lib, kind, steps, bar):
     l2 = np.linalg.norm(ug - uo) / np.linalg.norm(uo)
     print("u L2 rel", l2)
     assert l2 <= bar
+
+
+@pytest.mark.parametrize("bs", ["64", "256", "512"])
+@pytest.mark.parametrize("kind,cm,Nx,Ny", [("channel_convective", 0, 40, 24), ("ldc", 1, 40, 24),
+                                           ("periodic_force", 0, 37, 64), ("shear_velocity", 1, 9, 4),
+                                           ("channel_pressure", 0, 131, 516)])
+def test_fp32_push_kernels_bitwise(gpu_lib, monkeypatch, kind, cm, Nx, Ny, bs):
+    """The four-sites-per-lane stream-collide (k_push32x4_lds, default when
+    Ny % 4 == 0) equals the one-site-per-lane LDS kernel bit for bit, at
+    block sizes whose last block is partial and rows shorter and longer than
+    a block, with every boundary type and both collision operators."""
+    from life_amd.case import Case
+    p, g = bc_mix_params(kind, Nx=Nx, Ny=Ny, cm=cm)
+    out = []
+    for kern in ("lds", "x4lds"):
+        monkeypatch.setenv("LIFE_PUSH32", kern)
+        monkeypatch.setenv("LIFE_PUSH32_X4BS", bs)
+        c = Case(p, g, precision=1)
+        c.step(40)
+        out.append(c.fields())
+        c.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
