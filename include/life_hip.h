@@ -71,12 +71,13 @@ const char *life_last_error(void);
 const char *life_version(void);
 /* Number of visible HIP devices (0 without a GPU). */
 int life_device_count(int *count);
-/* Free the library's pooled HIP resources: idle streams and the device and
- * page-locked blocks kept for reuse by later grids, and -- when no grid or
- * server is live -- the shared CU-masked streams.  The last destroy of a
- * grid or server does the same by itself; a host calls this before it
- * exits with handles it never destroyed.  No reference counterpart (the
- * reference allocates no device resources). */
+/* Free the library's pooled HIP resources: the device and page-locked
+ * blocks kept for reuse by later grids and -- when no grid or server is
+ * live -- the pooled and CU-masked streams, which otherwise live until the
+ * process ends.  The last destroy of a grid or server frees the blocks by
+ * itself; a host calls this once before it exits (the Python package does
+ * so at interpreter exit).  No reference counterpart (the reference
+ * allocates no device resources). */
 int life_release_cached(void);
 
 /* ---------------------------------------------------------------- layer 1 */

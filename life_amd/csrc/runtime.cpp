@@ -18,8 +18,9 @@
 namespace life {
 
 namespace {
-// Freed pinned blocks by size (never destroyed: the blocks stay allocated
-// until the process ends, and no HIP call runs from static destructors).
+// Freed pinned blocks by size (the cache object is never destroyed, so no
+// HIP call runs from static destructors; its blocks are freed when the
+// last grid or server goes).
 struct PinnedCache {
     std::mutex mu;
     std::map<size_t, std::vector<void*>> blocks;
@@ -530,7 +531,8 @@ static void free_markers(life_grid* g) {
 
 namespace {
 // CU-masked streams, one per (device, reserve), shared by every grid and
-// server on the device while any handle is live.
+// server on the device, created on first use and kept until the host's
+// final teardown (life_release_cached).
 struct MaskedStreams {
     std::mutex mu;
     std::map<std::pair<int, int>, hipStream_t> streams;
@@ -540,9 +542,8 @@ MaskedStreams& masked_streams() {
     return *m;
 }
 
-// Live grids and servers.  When the last one goes, every pooled HIP
-// resource is released (release_cached), so that nothing the library owns
-// is left for the HIP runtime's own teardown at process exit.
+// Live grids and servers.  When the last one goes, the pooled memory is
+// released; the streams stay for the next grid or server.
 std::mutex g_handles_mu;
 int g_handles = 0;
 }  // namespace
@@ -552,9 +553,9 @@ int g_handles = 0;
 // round-robin over the XCDs: CU x*G + ((x + 8*k) mod G) for XCD x and
 // k = 0, 1, ... (G = CUs per XCD) falls on XCD x either way.  A CU-masked
 // stream has a hardware queue of its own, so there is one per device and
-// reserve, shared by every grid or server on it and destroyed when the
-// last of them is.  Sharing cannot deadlock: a user records the event its
-// kernel waits for before it queues that wait.
+// reserve for the process, shared by every grid or server on it.  Sharing
+// cannot deadlock: a user records the event its kernel waits for before it
+// queues that wait.
 hipStream_t masked_stream(int device, int reserve_cus) {
     MaskedStreams& ms = masked_streams();
     std::lock_guard<std::mutex> lock(ms.mu);
@@ -581,16 +582,16 @@ hipStream_t masked_stream(int device, int reserve_cus) {
     return st;
 }
 
-// Frees the pools: idle grid streams, kept device and pinned blocks, and,
-// with shared_streams, the CU-masked streams (only when no grid or server
-// is live: the caller holds g_handles_mu and checked).
-static void release_pools(bool shared_streams) {
-    {
-        std::lock_guard<std::mutex> lock(g_stream_mu);
-        for (auto& kv : g_free_streams)
-            for (hipStream_t st : kv.second) (void)hipStreamDestroy(st);
-        g_free_streams.clear();
-    }
+// Frees the pooled memory (kept device and page-locked blocks) and, with
+// streams, the idle grid streams and the CU-masked streams too (only when
+// no grid or server is live: the caller holds g_handles_mu and checked).
+// Streams are released only at the host's final teardown
+// (life_release_cached), never when the last handle goes: destroying a
+// CU-masked stream and creating the next one within a process (every time
+// a test or a host closed its last case and opened another) stalled the
+// next life_server_create, inside stream creation, for minutes; a stream
+// created once per (device, reserve) for the process does not.
+static void release_pools(bool streams) {
     {
         DeviceCache& dc = device_cache();
         std::lock_guard<std::mutex> lock(dc.mu);
@@ -600,15 +601,20 @@ static void release_pools(bool shared_streams) {
         dc.bytes = 0;
     }
     pinned_release_all();
-    if (shared_streams) {
-        MaskedStreams& ms = masked_streams();
-        std::lock_guard<std::mutex> lock(ms.mu);
-        for (auto& kv : ms.streams) {
-            (void)hipStreamSynchronize(kv.second);
-            (void)hipStreamDestroy(kv.second);
-        }
-        ms.streams.clear();
+    if (!streams) return;
+    {
+        std::lock_guard<std::mutex> lock(g_stream_mu);
+        for (auto& kv : g_free_streams)
+            for (hipStream_t st : kv.second) (void)hipStreamDestroy(st);
+        g_free_streams.clear();
     }
+    MaskedStreams& ms = masked_streams();
+    std::lock_guard<std::mutex> lock(ms.mu);
+    for (auto& kv : ms.streams) {
+        (void)hipStreamSynchronize(kv.second);
+        (void)hipStreamDestroy(kv.second);
+    }
+    ms.streams.clear();
 }
 
 void handle_acquire() {
@@ -618,7 +624,7 @@ void handle_acquire() {
 
 void handle_release() {
     std::lock_guard<std::mutex> lock(g_handles_mu);
-    if (--g_handles == 0) release_pools(true);
+    if (--g_handles == 0) release_pools(false);
 }
 
 }  // namespace life

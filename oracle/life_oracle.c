@@ -1633,12 +1633,51 @@ void orc_body_info(const orc_case *c, int b, int *info) {
     info[4] = B->fem ? B->fem->ndof : 0;
 }
 
+/* Body b's shape (0 circle, 1 filament) and its markers' global indices in
+   the body's own node order (iBody[ib].node, IBMBody.h). */
+int orc_body_nodes(const orc_case *c, int b, int *idx) {
+    const body_t *B = &c->b[b];
+    if (idx) memcpy(idx, B->nodes, sizeof(int) * B->n);
+    return B->kind;
+}
+
 int orc_get_fem(const orc_case *c, int b, double *U, double *Udot, double *Udotdot) {
     const fem_t *F = c->b[b].fem;
     if (!F) return -1;
     if (U) memcpy(U, F->U, sizeof(double) * F->ndof);
     if (Udot) memcpy(Udot, F->Ud, sizeof(double) * F->ndof);
     if (Udotdot) memcpy(Udotdot, F->Udd, sizeof(double) * F->ndof);
+    return F->ndof;
+}
+
+/* The rest of a flexible body's restart state (FEM.restart as
+   ObjectsClass::writeRestart lays it out, Objects.cpp:1290-1352): U_n,
+   U_nm1, R_k (ndof each), (L, angle) per element and (pos0, pos, angle0,
+   angle) per node.  sizes: nel, nnode. */
+int orc_get_fem_state(const orc_case *c, int b, int *sizes, double *Un, double *Unm1, double *Rk, double *el,
+                      double *nodes) {
+    const fem_t *F = c->b[b].fem;
+    if (!F) return -1;
+    sizes[0] = F->nel;
+    sizes[1] = F->nnode;
+    if (Un) memcpy(Un, F->Un, sizeof(double) * F->ndof);
+    if (Unm1) memcpy(Unm1, F->Unm1, sizeof(double) * F->ndof);
+    if (Rk) memcpy(Rk, F->Rk, sizeof(double) * F->ndof);
+    if (el)
+        for (int e = 0; e < F->nel; e++) {
+            el[2 * e] = F->el[e].L;
+            el[2 * e + 1] = F->el[e].angle;
+        }
+    if (nodes)
+        for (int n = 0; n < F->nnode; n++) {
+            double *q = nodes + 6 * n;
+            q[0] = F->node[n].pos0[0];
+            q[1] = F->node[n].pos0[1];
+            q[2] = F->node[n].pos[0];
+            q[3] = F->node[n].pos[1];
+            q[4] = F->node[n].angle0;
+            q[5] = F->node[n].angle;
+        }
     return F->ndof;
 }
 

@@ -70,7 +70,10 @@ void orc_get_markers(const orc_case *c, double *pos, double *vel, double *force,
                      double *eps, double *ds, int *body);
 void orc_get_supports(const orc_case *c, int *count, int *ij, double *delta);
 void orc_body_info(const orc_case *c, int b, int *info /* ID, flex, nmarkers, first, ndof */);
+int orc_body_nodes(const orc_case *c, int b, int *idx /* n */);
 int orc_get_fem(const orc_case *c, int b, double *U, double *Udot, double *Udotdot);
+int orc_get_fem_state(const orc_case *c, int b, int *sizes /* nel, nnode */, double *Un, double *Unm1,
+                      double *Rk, double *el /* 2 nel */, double *nodes /* 6 nnode */);
 void orc_coupling_info(const orc_case *c, double *out /* subIt, subRes, relax */);
 
 /* writeTotalForces quantities (Objects.cpp:491-539): Fx, Fy, Cx, Cy. */

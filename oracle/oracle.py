@@ -72,6 +72,8 @@ def lib():
         L.orc_get_supports.argtypes = [vp, ip, ip, dp]
         L.orc_body_info.argtypes = [vp, C.c_int, ip]
         L.orc_get_fem.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.orc_body_nodes.argtypes = [vp, C.c_int, ip]
+        L.orc_get_fem_state.argtypes = [vp, C.c_int, ip, dp, dp, dp, dp, dp]
         L.orc_coupling_info.argtypes = [vp, dp]
         L.orc_total_force.argtypes = [vp, dp]
         L.orc_dirac_delta.restype = C.c_double
@@ -218,11 +220,30 @@ class OracleCase:
         lib().orc_body_info(self.h, int(b), _ip(info))
         return dict(ID=int(info[0]), flex=int(info[1]), n=int(info[2]), first=int(info[3]), ndof=int(info[4]))
 
+    def body_nodes(self, b):
+        """(shape, marker indices): shape 0 circle, 1 filament; the body's
+        markers in its own node order."""
+        idx = np.zeros(self.body_info(b)["n"], np.int32)
+        kind = lib().orc_body_nodes(self.h, int(b), _ip(idx))
+        return int(kind), idx
+
     def fem(self, b):
         nd = self.body_info(b)["ndof"]
         U, Ud, Udd = np.zeros(nd), np.zeros(nd), np.zeros(nd)
         lib().orc_get_fem(self.h, int(b), _dp(U), _dp(Ud), _dp(Udd))
         return U, Ud, Udd
+
+    def fem_state(self, b):
+        """U_n, U_nm1, R_k, per-element (L, angle) and per-node (pos0, pos,
+        angle0, angle) of flexible body b."""
+        nd = self.body_info(b)["ndof"]
+        sizes = np.zeros(2, np.int32)
+        lib().orc_get_fem_state(self.h, int(b), _ip(sizes), None, None, None, None, None)
+        nel, nnode = int(sizes[0]), int(sizes[1])
+        Un, Unm1, Rk = np.zeros(nd), np.zeros(nd), np.zeros(nd)
+        el, nodes = np.zeros((nel, 2)), np.zeros((nnode, 6))
+        lib().orc_get_fem_state(self.h, int(b), _ip(sizes), _dp(Un), _dp(Unm1), _dp(Rk), _dp(el), _dp(nodes))
+        return dict(U_n=Un, U_nm1=Unm1, R_k=Rk, elements=el, nodes=nodes)
 
     def coupling(self):
         out = np.zeros(3)
