@@ -127,6 +127,12 @@ int life_ibm_find_support(life_grid *g, int flexible_only);
 /* Epsilon matrix of ObjectsClass::computeEpsilon (Objects.cpp:258-302) for
  * the system made of markers [first, first+n): A row-major n x n.  */
 int life_ibm_epsilon_matrix(life_grid *g, int first, int n, double *A);
+/* life_ibm_epsilon_matrix without any device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (stream
+ * capture, tight serving loops).  The matrix is assembled in a fixed 8 MiB
+ * device array per GPU (n <= 1024 markers; larger n returns an error) that
+ * every call on that GPU shares: overlapping calls on one GPU race.  */
+int life_ibm_epsilon_matrix_noalloc(life_grid *g, int first, int n, double *A);
 /* Set epsilon for all markers (Objects.cpp:309-310). */
 int life_ibm_set_epsilon(life_grid *g, const double *eps);
 /* ObjectsClass::ibmKernelInterp (Objects.cpp:102-117): interpolate rho and

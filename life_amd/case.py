@@ -92,6 +92,12 @@ class Grid:
         L.check(L.load().life_ibm_epsilon_matrix(self.h, int(first), int(n), L.dptr(A)))
         return A
 
+    def epsilon_matrix_noalloc(self, first, n):
+        """The same matrix through the allocation-free entry point."""
+        A = np.zeros((n, n))
+        L.check(L.load().life_ibm_epsilon_matrix_noalloc(self.h, int(first), int(n), L.dptr(A)))
+        return A
+
     def set_epsilon(self, eps):
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         L.check(L.load().life_ibm_set_epsilon(self.h, L.dptr(eps)))

@@ -82,6 +82,7 @@ SIGNATURES = {
     "life_ibm_update_markers": (C.c_int, [_vp, _dp, _dp]),
     "life_ibm_find_support": (C.c_int, [_vp, C.c_int]),
     "life_ibm_epsilon_matrix": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "life_ibm_epsilon_matrix_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "life_ibm_set_epsilon": (C.c_int, [_vp, _dp]),
     "life_ibm_interpolate": (C.c_int, [_vp, _dp]),
     "life_ibm_spread": (C.c_int, [_vp]),
