@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "fem.h"
+#include "fem_dev.h"
 #include "runtime.h"
 
 namespace life {
@@ -111,6 +112,12 @@ struct life_case {
     bool forces_on_host = false;
     std::unique_ptr<life::PhaseTimes> timing{new life::PhaseTimes()};
     std::vector<hipEvent_t> kev;  // per-step stream-collide events of life_case_step_timed
+    // FEM of the flexible bodies on the device (FP32-shifted populations;
+    // empty otherwise).  While it runs the steps, the host's FemBody state
+    // and marker arrays lag behind the device (host_stale) until a reader
+    // calls case_sync_host.
+    life::FemDevCase fdev;
+    bool host_stale = false;
 
     life::MarkerView view(const life::Body& b) {
         return life::MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};
@@ -129,6 +136,12 @@ void case_recompute_host(life_case* c);
 // femKernel with the forces and epsilon in c->force, c->eps; updates
 // subRes/subNum/subDen and increments subIt
 void case_fem(life_case* c);
+// Aitken factor of subiteration subIt >= 1 (Objects.cpp:179-190), in c->relax
+void case_aitken(life_case* c);
+// host FEM state and marker arrays brought level with the device FEM
+void case_sync_host(life_case* c);
+// device FEM state level with the host's (after the host rewrote it)
+void case_push_fem(life_case* c);
 // another subiteration is due (Objects.cpp:59)
 inline bool case_subiterate(const life_case* c) { return c->subIt < kMaxSubIt && c->subRes > c->p.subTol; }
 // computeEpsilon for markers [first, first+n) by the dense path

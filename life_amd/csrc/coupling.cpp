@@ -224,6 +224,15 @@ void for_flexible(life_case* c, Fn&& fn) {
 
 }  // namespace
 
+void life::case_aitken(life_case* c) {
+    if (c->subIt == 1) {
+        const double a = std::fabs(c->relax), r = a < c->p.relaxMax ? a : c->p.relaxMax;
+        c->relax = (double)(((0.0 < c->relax) - (c->relax < 0.0)) * r);
+    } else {
+        c->relax = -c->relax * c->subNum / c->subDen;
+    }
+}
+
 // Host part of ObjectsClass::recomputeObjectVals, Objects.cpp:152-232.
 void life::case_recompute_host(life_case* c) {
     if (c->subIt == 0) {
@@ -232,15 +241,36 @@ void life::case_recompute_host(life_case* c) {
             b.fem->predictor(c->t, c->view(b));
         });
     } else {
-        if (c->subIt == 1) {
-            const double a = std::fabs(c->relax), r = a < c->p.relaxMax ? a : c->p.relaxMax;
-            c->relax = (double)(((0.0 < c->relax) - (c->relax < 0.0)) * r);
-        } else {
-            c->relax = -c->relax * c->subNum / c->subDen;
-        }
+        case_aitken(c);
         const double w = c->relax;
         for_flexible(c, [&](Body& b) { b.fem->relax(w, c->view(b)); });
     }
+}
+
+void life::case_sync_host(life_case* c) {
+    if (!c->host_stale) return;
+    life_grid* g = c->g;
+    LIFE_HIP(hipSetDevice(g->device));
+    lbm_join(g);
+    std::vector<FemBody*> fb;
+    for (Body* b : c->flex_bodies) fb.push_back(b->fem.get());
+    fem_dev_download(c->fdev, fb, g->stream);
+    const size_t b2 = sizeof(double) * 2 * c->nm, b1 = sizeof(double) * c->nm;
+    LIFE_HIP(hipMemcpyAsync(c->pos.data(), g->md.pos, b2, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->vel.data(), g->md.vel, b2, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->force.data(), g->md.force, b2, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->eps.data(), g->md.eps, b1, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipMemcpyAsync(c->ds.data(), g->md.ds, b1, hipMemcpyDeviceToHost, g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    c->host_stale = false;
+}
+
+void life::case_push_fem(life_case* c) {
+    if (!c->fdev.ok()) return;
+    std::vector<const FemBody*> fb;
+    for (Body* b : c->flex_bodies) fb.push_back(b->fem.get());
+    fem_dev_upload(c->fdev, fb, c->g->stream);
+    c->host_stale = false;
 }
 
 // ObjectsClass::femKernel, Objects.cpp:63-99 (ORDERED: sums in body order).
@@ -315,7 +345,63 @@ void launch_pending(life_case* c, EarlyBulk& eb) {
     eb.want = false;
 }
 
+// objectKernel with the FEM on the device: per subiteration the host only
+// reads the three residual sums (and the out-of-band flags of epsilon).
+void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
+    life_grid* g = c->g;
+    FemDevCase& fd = c->fdev;
+    PhaseTimes& T = *c->timing;
+    c->subIt = 0;
+    c->forces_on_host = false;
+    bool more = false;
+    do {
+        // recomputeObjectVals: predictor or relaxation moves the markers
+        if (c->subIt == 0) {
+            launch_fem_predict(fd.d_bodies, nullptr, fd.nb, c->t, nullptr, fd.lds, g->stream);
+        } else {
+            case_aitken(c);
+            launch_fem_relax(fd.d_bodies, nullptr, fd.nb, c->relax, nullptr, fd.lds, g->stream);
+        }
+        LIFE_HIP(hipGetLastError());
+        ibm_find_support(g, 1);
+        compute_epsilon(c, false);
+        T.lap(g, 1);
+        ibm_interpolate(g);
+        if (mode == 2) launch_pending(c, eb);
+        T.lap(g, 2);
+        // femKernel; again with epsilon from the dense path if a system was
+        // found not to be banded
+        bool redo = false;
+        for (;;) {
+            launch_fem_dynamic(fd.d_bodies, nullptr, fd.nb, fd.lds, redo, g->stream);
+            launch_fem_sums(fd.d_bodies, fd.nb, fd.sums, g->stream);
+            LIFE_HIP(hipGetLastError());
+            LIFE_HIP(hipMemcpyAsync(fd.sums_h, fd.sums, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+            LIFE_HIP(hipEventRecord(c->ev_forces, g->stream));
+            LIFE_HIP(hipEventSynchronize(c->ev_forces));
+            if (!finish_epsilon(c)) break;
+            redo = true;
+        }
+        T.subits++;
+        T.lap(g, 3);
+        // FEMBodyClass::subResidual summed over the bodies (case_fem)
+        c->subRes = std::sqrt(fd.sums_h[0]) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
+        c->subNum = fd.sums_h[1];
+        c->subDen = fd.sums_h[2];
+        c->subIt++;
+        more = case_subiterate(c);
+    } while (more);
+    ibm_spread(g);
+    T.lap(g, 4);
+    c->host_stale = true;
+    if (c->subIt == kMaxSubIt) throw std::runtime_error("Subiteration scheme hit 20 iterations");
+}
+
 void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
+    if (c->fdev.ok()) {
+        object_kernel_dev(c, eb, mode);
+        return;
+    }
     c->subIt = 0;
     c->forces_on_host = false;
     PhaseTimes& T = *c->timing;
@@ -370,6 +456,10 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
 // switch it between calls.
 constexpr long long kOverlapMinSites = 1LL << 22;
 int overlap_mode(const life_case* c) {
+    // a per-site force (WOMERSLEY with gravity) lives in two planes that
+    // alternate by step: an early stream-collide would overwrite the one
+    // this step's coupling reads
+    if (c->g->fsite[0]) return 0;
     const char* e = std::getenv("LIFE_OVERLAP");
     if (e) {
         const int v = std::atoi(e);
@@ -405,6 +495,30 @@ void case_blowup_cadence(life_case* c) {
     if (g->check_every <= 0 || c->t % g->check_every != 0) return;
     lbm_join(g);
     blowup_cadence(g, c->t, g->stream);
+}
+
+// The FEM runs on the device for FP32-shifted populations when the
+// environment sets LIFE_DEVICE_FEM = 1; the plain method always keeps the
+// host FEM, whose transcendental functions match the oracle's bit for bit.
+bool device_fem_wanted(int precision) {
+    if (precision != LIFE_FP32_SHIFTED) return false;
+    const char* e = std::getenv("LIFE_DEVICE_FEM");
+    return e && std::atoi(e) != 0;
+}
+
+void setup_device_fem(life_case* c) {
+    std::vector<const FemBody*> fb;
+    std::vector<int> first, nm;
+    for (Body* b : c->flex_bodies) {
+        if (fem_dev_lds_bytes(b->fem->ndof()) > kFemMaxLds) return;  // too large: host FEM
+        fb.push_back(b->fem.get());
+        first.push_back(b->first);
+        nm.push_back(b->n);
+    }
+    const life_params* p = &c->p;
+    const FemDevScalars s{c->cs.Dx, c->cs.Dt, c->cs.Dm, p->alpha, p->delta, p->gravityX, p->gravityY, p->ref_L};
+    life_grid* g = c->g;
+    fem_dev_create(c->fdev, fb, first, nm, g->md.pos, g->md.vel, g->md.force, g->md.eps, s, g->stream);
 }
 
 }  // namespace
@@ -458,6 +572,7 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
                                     c->g->stream));
             LIFE_HIP(hipStreamSynchronize(c->g->stream));
             check_flags(c->g);
+            if (c->hasFlex && device_fem_wanted(precision)) setup_device_fem(c);
         }
     });
     if (rc) {
@@ -477,6 +592,8 @@ int life_case_destroy(life_case* c) {
         for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
         if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
         if (c->ev_forces) (void)hipEventDestroy(c->ev_forces);
+        (void)hipStreamSynchronize(c->g->stream);
+        fem_dev_destroy(c->fdev);
         rc = life_grid_destroy(c->g);
     }
     delete c;
@@ -578,6 +695,7 @@ int life_case_body_info(const life_case* c, int b, int info[5]) {
 int life_case_get_fem(const life_case* c, int b, double* U, double* Udot, double* Udotdot) {
     return guarded([&] {
         if (b < 0 || b >= (int)c->bodies.size() || !c->bodies[b].fem) throw std::runtime_error("not a flexible body");
+        case_sync_host(const_cast<life_case*>(c));
         const FemBody& F = *c->bodies[b].fem;
         const size_t n = (size_t)F.ndof() * sizeof(double);
         if (U) std::memcpy(U, F.U.data(), n);
@@ -597,6 +715,7 @@ int life_case_total_force(life_case* c, double out[4]) {
     return guarded([&] {
         life_grid* g = c->g;
         LIFE_HIP(hipSetDevice(g->device));
+        case_sync_host(c);
         if (c->nm) {
             LIFE_HIP(hipMemcpyAsync(c->force.data(), g->md.force, sizeof(double) * 2 * c->nm, hipMemcpyDeviceToHost,
                                     g->stream));

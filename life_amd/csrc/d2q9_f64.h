@@ -60,6 +60,11 @@ LIFE_DI void moments64(const double fs[kNV], double Fx, double Fy, double& rho, 
     uy = (my + 0.5 * Fy) / r;
 }
 
+// force_xy of one site from its density (Grid.cpp:59-60), same operation
+// order: (rho Drho g + dp cos) (Dx Dt)^2 / Dm.
+LIFE_DI double site_force_x(const GridScalars& g, double rho) { return (rho * g.fDrho * g.fgx + g.fdx) * g.fsc / g.fDm; }
+LIFE_DI double site_force_y(const GridScalars& g, double rho) { return (rho * g.fDrho * g.fgy + g.fdy) * g.fsc / g.fDm; }
+
 LIFE_DI double density64(const double fs[kNV]) {
     double r = 0.0;
 #pragma unroll

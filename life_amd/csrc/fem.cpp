@@ -173,6 +173,58 @@ void FemBody::set_node_state(int n, const double in[6]) {
     d.angle = in[5];
 }
 
+void FemBody::layout(Layout& out) const {
+    out = Layout{};
+    for (const Node& n : node_) {
+        for (double v : {n.pos0[0], n.pos0[1], n.pos[0], n.pos[1], n.angle0, n.angle}) out.node.push_back(v);
+    }
+    out.map_start.push_back(0);
+    for (const Elem& e : el_) {
+        for (double v : {e.L0, e.L, e.angle, e.A, e.I, e.E, e.rho}) out.el.push_back(v);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                out.Mloc.push_back(e.M[i][j]);
+                out.KLloc.push_back(e.KL[i][j]);
+            }
+        for (const Map& m : e.map) {
+            out.map_node.push_back(m.node);
+            out.map_z.push_back(m.z1);
+            out.map_z.push_back(m.z2);
+        }
+        out.map_start.push_back((int)out.map_node.size());
+    }
+    for (const PosMap& pm : pmap_) {
+        out.pm_el.push_back(pm.el);
+        out.pm_zeta.push_back(pm.zeta);
+    }
+}
+
+void FemBody::import_state(const double* vecs, const double* node, const double* el, double subRes, double subNum,
+                           double subDen, int itNR, double resNR) {
+    std::vector<double>* v[] = {&U, &Ud, &Udd, &Un, &Unm1, &Unm2, &Udn, &Uddn, &Ukm1, &Rk, &Rkm1};
+    for (int k = 0; k < 11; k++) std::memcpy(v[k]->data(), vecs + (size_t)k * ndof_, sizeof(double) * ndof_);
+    for (size_t n = 0; n < node_.size(); n++) {
+        Node& d = node_[n];
+        const double* q = node + 6 * n;
+        d.pos0[0] = q[0];
+        d.pos0[1] = q[1];
+        d.pos[0] = q[2];
+        d.pos[1] = q[3];
+        d.angle0 = q[4];
+        d.angle = q[5];
+    }
+    for (size_t e = 0; e < el_.size(); e++) {
+        el_[e].L = el[7 * e + 1];
+        el_[e].angle = el[7 * e + 2];
+        transform(el_[e]);
+    }
+    subRes_ = subRes;
+    subNum_ = subNum;
+    subDen_ = subDen;
+    itNR_ = itNR;
+    resNR_ = resNR;
+}
+
 void FemBody::transform(Elem& e) {
     const double ca = std::cos(e.angle), sa = std::sin(e.angle);
     e.T[0][0] = e.T[1][1] = e.T[3][3] = e.T[4][4] = ca;

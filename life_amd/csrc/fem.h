@@ -74,6 +74,22 @@ class FemBody {
     void set_element_state(int e, double L, double angle);
     void set_node_state(int n, const double in[6]);
 
+    // Flat copy of the body for the device FEM (fem_dev.hip): per node
+    // pos0[2], pos[2], angle0, angle; per element L0, L, angle, A, I, E,
+    // rho; the local mass and linear stiffness matrices (constant); the
+    // element-to-marker load map (CSR: per element its entries' marker,
+    // z1, z2) and the marker-to-element map (element, zeta).
+    struct Layout {
+        std::vector<double> node, el, Mloc, KLloc, map_z, pm_zeta;
+        std::vector<int> map_start, map_node, pm_el;
+    };
+    void layout(Layout& out) const;
+    // Back from the device: the vectors (U, Ud, Udd, Un, Unm1, Unm2, Udn,
+    // Uddn, Ukm1, Rk, Rkm1 in that order, ndof each), node and element
+    // state in the Layout's order, and the last solve's scalars.
+    void import_state(const double* vecs, const double* node, const double* el, double subRes, double subNum,
+                      double subDen, int itNR, double resNR);
+
   private:
     struct Node {
         double pos0[2], pos[2], angle0, angle;

@@ -638,7 +638,7 @@ __device__ inline void interpolate_slots(const T* __restrict__ f, const MarkerDe
         const long long id = m.supp_site[k * kSupp + s];
         double fs[kNV], rho, ux, uy;
         load_site(f, g.S, id, fs);
-        moments64(fs, g.Fx, g.Fy, rho, ux, uy);
+        moments64(fs, g.fcur ? g.fcur[id] : g.Fx, g.fcur ? g.fcur[g.S + id] : g.Fy, rho, ux, uy);
         const double d = m.supp_delta[k * kSupp + s];
         a[t] = rho * d * 1.0 * 1.0;
         bx[t] = rho * ux * d * 1.0 * 1.0;

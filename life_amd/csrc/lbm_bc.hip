@@ -111,7 +111,8 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
         } else {
             double fp[kNV];
             load_site(src, S, id, fp);
-            moments64(fp, g.Fxm + fibm[id], g.Fym + fibm[S + id], rn, uxn, uyn);
+            moments64(fp, (g.fprev ? g.fprev[id] : g.Fxm) + fibm[id], (g.fprev ? g.fprev[S + id] : g.Fym) + fibm[S + id],
+                      rn, uxn, uyn);
         }
     }
     const long long id1 = (long long)(b.i + b.nx) * Ny + (j + b.ny);
@@ -138,8 +139,8 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
             double f1[kNV], f2[kNV], r1, r2, u1x, u1y, u2x, u2y;
             load_site(dst, S, id1, f1);
             load_site(dst, S, id2, f2);
-            moments64(f1, g.Fx, g.Fy, r1, u1x, u1y);
-            moments64(f2, g.Fx, g.Fy, r2, u2x, u2y);
+            moments64(f1, g.fcur ? g.fcur[id1] : g.Fx, g.fcur ? g.fcur[S + id1] : g.Fy, r1, u1x, u1y);
+            moments64(f2, g.fcur ? g.fcur[id2] : g.Fx, g.fcur ? g.fcur[S + id2] : g.Fy, r2, u2x, u2y);
             if (b.nd != 0) uxn = (4.0 / 3.0) * u1x - (1.0 / 3.0) * u2x;
             if (b.nd != 1) uyn = (4.0 / 3.0) * u1y - (1.0 / 3.0) * u2y;
             break;
@@ -207,7 +208,8 @@ __device__ inline void convective_column(const T* __restrict__ src, const GridSc
     } else {
         double fs[kNV], r;
         load_site(src, S, id, fs);
-        moments64(fs, g.Fxm + fibm[id], g.Fym + fibm[S + id], r, ux, uy);
+        moments64(fs, (g.fprev ? g.fprev[id] : g.Fxm) + fibm[id], (g.fprev ? g.fprev[S + id] : g.Fym) + fibm[S + id],
+                  r, ux, uy);
     }
     ucol[(c * 2 + 0) * (long long)Ny + j] = ux;
     ucol[(c * 2 + 1) * (long long)Ny + j] = uy;
@@ -344,7 +346,8 @@ __global__ __launch_bounds__(256) void k_materialise(const T* __restrict__ f, Gr
             ux = u0[2 * id];
             uy = u0[2 * id + 1];
         } else {
-            moments64(fs, g.Fx + fibm[id], g.Fy + fibm[g.S + id], r, ux, uy);
+            moments64(fs, (g.fcur ? g.fcur[id] : g.Fx) + fibm[id], (g.fcur ? g.fcur[g.S + id] : g.Fy) + fibm[g.S + id],
+                      r, ux, uy);
         }
         if (rho) rho[id] = r;
         if (u) {

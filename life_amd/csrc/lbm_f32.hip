@@ -342,8 +342,11 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
 
 // Launch configuration of the bulk stream-collide kernel, read once from
 // the environment for tuning sweeps: LIFE_PUSH32 = scalar | x4 | lds |
-// x4lds (kernel; x4lds, the default, needs Ny % 4 == 0 and falls back to lds
-// otherwise), LIFE_PUSH32_X4BS = 64 | 128 | 256 | 512 (lanes per block of
+// x4lds (kernel; lds is the default: on the 8192 x 4096 canopy it measured
+// 0.438 ms alone against 0.452-0.459 ms for x4lds at every block size, and
+// the small x4lds blocks delay the coupling kernels beside it; x4lds needs
+// Ny % 4 == 0 and falls back to lds otherwise), LIFE_PUSH32_X4BS = 64 |
+// 128 | 256 | 512 (lanes per block of
 // k_push32x4_lds), LIFE_PUSH32_NT = 0..7 (streaming-hint mask of k_push32),
 // LIFE_PUSH32_XCD = 0 | 1 (block remap), LIFE_PUSH32_BS = 64 | 128 | 256
 // (block size of k_push32), LIFE_PUSH32_LDSBS = 256 | 512 | 1024 (block size
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(256) void k_support_push32(const float* __restrict_
 // (streaming stores in k_push32_lds).
 struct PushConfig {
     bool x4 = false;
-    bool x4lds = true;
+    bool x4lds = false;
     int x4lds_bs = 256;
     bool lds = true;
     int nt = 3;

@@ -39,8 +39,22 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
         rho = rho0[id];
         ux = u0[2 * id];
         uy = u0[2 * id + 1];
-        Fx = g.Fx + fibm[id];
-        Fy = g.Fy + fibm[S + id];
+        double sx = g.Fx, sy = g.Fy;
+        if (g.fcur) {
+            sx = site_force_x(g, rho);
+            sy = site_force_y(g, rho);
+            g.fcur[id] = sx;
+            g.fcur[S + id] = sy;
+        }
+        Fx = sx + fibm[id];
+        Fy = sy + fibm[S + id];
+    } else if (g.fcur) {
+        moments64(fs, g.fprev[id], g.fprev[S + id], rho, ux, uy);
+        const double sx = site_force_x(g, rho), sy = site_force_y(g, rho);
+        g.fcur[id] = sx;
+        g.fcur[S + id] = sy;
+        Fx = sx + 0.0;
+        Fy = sy + 0.0;
     } else {
         moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
     }
@@ -61,8 +75,9 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
 
 // Bulk step with LDS-realigned stores (push_store_realigned): the same
 // values reach the same addresses as k_push64, with aligned wavefront stores.
-// SP/DP: double pointers, generic or in the global address space.
-template <bool CM, typename SP, typename DP>
+// SP/DP: double pointers, generic or in the global address space.  SITEF:
+// force_xy per site (g.fprev / g.fcur), else the uniform g.Fxm / g.Fx0.
+template <bool CM, bool SITEF, typename SP, typename DP>
 __device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restrict__ dst, const GridScalars& g,
                                                  long long base, int t, PushStage<double>& st) {
     const long long id = base + t;
@@ -75,17 +90,27 @@ __device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restr
 #pragma unroll
     for (int v = 0; v < kNV; v++) fs[v] = valid ? __builtin_nontemporal_load(&src[v * S + idc]) : 1.0;
     double rho, ux, uy;
-    moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
-    collide64<CM>(g, fs, rho, ux, uy, g.Fx0, g.Fy0, fo);
+    if (SITEF) {
+        moments64(fs, g.fprev[idc], g.fprev[S + idc], rho, ux, uy);
+        const double sx = site_force_x(g, rho), sy = site_force_y(g, rho);
+        if (valid) {
+            g.fcur[idc] = sx;
+            g.fcur[S + idc] = sy;
+        }
+        collide64<CM>(g, fs, rho, ux, uy, sx + 0.0, sy + 0.0, fo);
+    } else {
+        moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
+        collide64<CM>(g, fs, rho, ux, uy, g.Fx0, g.Fy0, fo);
+    }
     push_store_realigned<double, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
 }
 
-template <bool CM>
+template <bool CM, bool SITEF>
 __global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
                                                            GridScalars g, unsigned nblocks) {
     __shared__ PushStage<double> st;
     const long long base = (long long)xcd_block(blockIdx.x, nblocks) * kPushBlock;
-    push64_lds_block<CM>(src, dst, g, base, threadIdx.x, st);
+    push64_lds_block<CM, SITEF>(src, dst, g, base, threadIdx.x, st);
 }
 
 // Push of one site with periodic wrap (Grid.cpp:226-244).
@@ -125,7 +150,7 @@ __global__ __launch_bounds__(kServeBlock64) void k_push64_batch(const CaseDev* _
         collide64<CM>(c.g, fs, c.rho0[id], c.u0[2 * id], c.u0[2 * id + 1], Fx, Fy, fo);
         push_site64(dst, c.g, id, fo);
     } else {
-        push64_lds_block<CM>((gptr<const double>)c.src, (gptr<double>)c.dst, c.g, base, threadIdx.x, st);
+        push64_lds_block<CM, false>((gptr<const double>)c.src, (gptr<double>)c.dst, c.g, base, threadIdx.x, st);
     }
 }
 
@@ -141,10 +166,12 @@ __device__ __forceinline__ void support_push64_site(const double* __restrict__ s
     load_site(src, S, id, fs);
     // moments as stored at the end of the previous step (its force_xy),
     // forcing of this step
-    const double Fxm = g.Fxm + fibm[id], Fym = g.Fym + fibm[S + id];
-    const double Fx = g.Fx + fibm[id], Fy = g.Fy + fibm[S + id];
+    const double Fxm = (g.fprev ? g.fprev[id] : g.Fxm) + fibm[id];
+    const double Fym = (g.fprev ? g.fprev[S + id] : g.Fym) + fibm[S + id];
     double rho, ux, uy;
     moments64(fs, Fxm, Fym, rho, ux, uy);
+    const double Fx = (g.fcur ? site_force_x(g, rho) : g.Fx) + fibm[id];
+    const double Fy = (g.fcur ? site_force_y(g, rho) : g.Fy) + fibm[S + id];
     collide64<CM>(g, fs, rho, ux, uy, Fx, Fy, fo);
     push_site64(dst, g, id, fo);
 }
@@ -176,8 +203,13 @@ void launch_push64(const double* src, double* dst, const GridScalars& g, const d
     }();
     if (!explicit_state && lds) {
         const unsigned nb = (unsigned)((g.N + kPushBlock - 1) / kPushBlock);
-        if (g.cm) k_push64_lds<true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
-        else k_push64_lds<false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        if (g.fcur) {
+            if (g.cm) k_push64_lds<true, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+            else k_push64_lds<false, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        } else {
+            if (g.cm) k_push64_lds<true, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+            else k_push64_lds<false, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        }
         return;
     }
     const unsigned nb = (unsigned)((g.N + 255) / 256);

@@ -51,6 +51,17 @@ struct GridScalars {
     double Fxm, Fym;        // force_xy of the previous step: the moments of the incoming populations
                             // (equal to Fx, Fy unless the forcing varies in time, WOMERSLEY)
     double Dx, velScale;    // Dx and Dt/Dx (IBMNode.cpp:54)
+    // force_xy per site (WOMERSLEY with gravity, Grid.cpp:52-61): null when
+    // the force is uniform.  Planes x at [id], y at [S + id].  fprev: the
+    // force of the previous step (the moments of the incoming populations);
+    // fcur: this step's, (rho_n Drho g + dp cos) Dx^2 Dt^2 / Dm from the
+    // incoming density rho_n, written by the bulk stream-collide and read by
+    // the kernels after it in the step.
+    const double* fprev;
+    double* fcur;
+    double fDrho, fgx, fgy;  // Drho and gravity (physical)
+    double fdx, fdy;         // dpdx, dpdy times this step's cosine
+    double fsc, fDm;         // (Dx Dt)^2 and Dm
 };
 
 }  // namespace life

@@ -89,29 +89,38 @@ inline void axpy_row(double* __restrict ri, const double* __restrict rk, double 
 }
 }  // namespace
 
+// Row operations touch only the columns a row can hold non-zeros in:
+// last[r] is the last such column of row r (its band, widened by the fill
+// of row swaps and updates).  Entries past it are exact zeros, so the
+// skipped operations (a - l * 0, s - 0 * x) would leave every value as it
+// is.  Without row swaps (the diagonally dominant FEM and epsilon systems)
+// each update covers ku columns instead of kl + ku.
 int lu_solve_band(const BandMatrix& A, int bc, const double* b, double* x, BandMatrix& W, std::vector<double>& y) {
     const int n = A.n, kl = A.kl, ku = A.ku;
     const int m = n - bc;
     if (W.n != m || W.kl != kl || W.ku != ku) W.resize(m, kl, ku);
-    W.zero();
     // row(r)[c] is entry (r, c) for c in [r - kl, r + kl + ku]
     const int w = W.w;
     double* const base = W.data.data();
     auto row = [&](int r) { return base + (ptrdiff_t)r * w + kl - r; };
     const double* const abase = A.data.data();
     auto arow = [&](int r) { return abase + (ptrdiff_t)r * A.w + kl - r; };
-    // copy the trailing block; entries beyond the original upper band start at zero
+    y.resize(2 * (size_t)m);
+    double* const yv = y.data();
+    int* const last = reinterpret_cast<int*>(yv + m);  // m ints in the second half
+    const int kext = kl + ku;
+    // copy the trailing block; the fill columns past the original upper band start at zero
     for (int r = 0; r < m; r++) {
         const int lo = r - kl < 0 ? 0 : r - kl;
         const int hi = r + ku > m - 1 ? m - 1 : r + ku;
+        const int top = r + kext > m - 1 ? m - 1 : r + kext;
         double* wr = row(r);
         const double* ar = arow(r + bc) + bc;
         for (int c = lo; c <= hi; c++) wr[c] = ar[c];
+        for (int c = hi + 1; c <= top; c++) wr[c] = 0.0;
+        last[r] = hi;
+        yv[r] = b[r + bc];
     }
-    y.resize(m);
-    double* const yv = y.data();
-    for (int r = 0; r < m; r++) yv[r] = b[r + bc];
-    const int kext = kl + ku;
     int info = 0;
     for (int k = 0; k < m; k++) {
         const int ilast = k + kl > m - 1 ? m - 1 : k + kl;
@@ -128,32 +137,37 @@ int lu_solve_band(const BandMatrix& A, int bc, const double* b, double* x, BandM
             if (!info) info = k + 1;
             continue;
         }
-        const int jlast = k + kext > m - 1 ? m - 1 : k + kext;
         double* rk = row(k);
         if (p != k) {
             double* rp = row(p);
-            for (int j = k; j <= jlast; j++) {
+            const int jl = last[k] > last[p] ? last[k] : last[p];
+            for (int j = k; j <= jl; j++) {
                 const double t = rk[j];
                 rk[j] = rp[j];
                 rp[j] = t;
             }
+            const int tl = last[k];
+            last[k] = last[p];
+            last[p] = tl;
             const double t = yv[k];
             yv[k] = yv[p];
             yv[p] = t;
         }
         const double piv = rk[k];
+        const int jlast = last[k];
         for (int i = k + 1; i <= ilast; i++) {
             double* ri = row(i);
             const double l = ri[k] / piv;
             ri[k] = l;
             axpy_row(ri, rk, l, k + 1, jlast);
+            if (last[i] < jlast) last[i] = jlast;
             yv[i] = yv[i] - l * yv[k];
         }
     }
     for (int i = m - 1; i >= 0; i--) {
         const double* ri = row(i);
         double s = yv[i];
-        const int jlast = i + kext > m - 1 ? m - 1 : i + kext;
+        const int jlast = last[i];
         for (int j = i + 1; j <= jlast; j++) s = s - ri[j] * x[j + bc];
         x[i + bc] = s / ri[i];
     }

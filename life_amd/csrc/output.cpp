@@ -489,6 +489,7 @@ extern "C" {
 
 int life_case_write_restart(life_case* c, const char* results) {
     return guarded_io([&] {
+        case_sync_host(c);
         const std::string dir = std::string(results) + "/Restart";
         fs::create_directories(dir);
         write_fluid_restart(c, dir);
@@ -501,8 +502,10 @@ int life_case_read_restart(life_case* c, const char* results) {
         const std::string base(results), dir = base + "/Restart";
         read_fluid_restart(c, dir);
         if (c->hasIBM) {
+            case_sync_host(c);  // nothing the device holds is newer than what the file replaces
             read_objects_restart(c, dir);
             case_restart_objects(c);
+            case_push_fem(c);
         }
         // ObjectsClass::restartCleanup, Objects.cpp:1387-1405
         const std::string vdir = base + "/VTK";

@@ -295,6 +295,27 @@ GridScalars step_scalars(const life_grid& g, int t) {
     s.Fy0 = F[1] + 0.0;
     s.Fxm = Fm[0];
     s.Fym = Fm[1];
+    if (g.fsite[0]) {
+        // gravity with WOMERSLEY: force_xy meets each site's density
+        // (Grid.cpp:59-60); this step writes plane t & 1 and reads the other
+        const life_params& p = g.p;
+        const CaseScalars& cs = g.cs;
+        double wave = 1.0;
+        if (t > 0) {
+            const double period = ((p.height_p * p.height_p) * M_PI) / (2.0 * (p.womersley * p.womersley) * p.nu_p);
+            wave = std::cos(2.0 * M_PI * t * cs.Dt / period);
+        }
+        const double DxDt = cs.Dx * cs.Dt;
+        s.fcur = g.fsite[t & 1];
+        s.fprev = g.fsite[(t + 1) & 1];
+        s.fDrho = cs.Drho;
+        s.fgx = p.gravityX;
+        s.fgy = p.gravityY;
+        s.fdx = p.dpdx * wave;
+        s.fdy = p.dpdy * wave;
+        s.fsc = DxDt * DxDt;
+        s.fDm = cs.Dm;
+    }
     return s;
 }
 
@@ -306,6 +327,7 @@ void lbm_join(life_grid* g) {
 
 void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (g->explicit_state) throw std::runtime_error("early stream-collide needs the populations' own moments");
+    if (g->fsite[0]) throw std::runtime_error("early stream-collide does not take a per-site force");
     const int src = g->cur, dst = 1 - g->cur;
     // populations final: the step's tail is already queued on stream2, or
     // the step ran on the grid's stream
@@ -722,9 +744,12 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         s.omega = g->cs.omega;
         s.cs2 = g->cs.cs2;
         s.cs4 = g->cs.cs4;
-        if (p->womersley != 0.0 && (p->gravityX != 0.0 || p->gravityY != 0.0))
-            throw std::runtime_error("WOMERSLEY forcing with gravity is not supported (force_xy would vary "
+        const bool site_force = p->womersley != 0.0 && (p->gravityX != 0.0 || p->gravityY != 0.0);
+        if (site_force && precision != LIFE_FP64)
+            throw std::runtime_error("WOMERSLEY forcing with gravity needs FP64 populations (force_xy varies "
                                      "from site to site)");
+        s.fcur = nullptr;
+        s.fprev = nullptr;
         double F[2];
         body_force(*g, 0, F);
         s.Fx = F[0];
@@ -735,6 +760,13 @@ int life_grid_create(const life_params* p, int precision, int device, life_grid*
         s.Fym = s.Fy;
         s.Dx = g->cs.Dx;
         s.velScale = g->cs.Dt / g->cs.Dx;
+        if (site_force) {
+            // force_xy of the previous and the current step, per site
+            for (int k = 0; k < 2; k++) {
+                g->fsite[k] = dalloc<double>(2 * (size_t)s.S);
+                launch_fill_f64(g->fsite[k], 0.0, 2 * s.S, g->stream);
+            }
+        }
 
         // boundary sites in the reference's i-major order (Grid.cpp:931-957)
         for (int i = 0; i < Nx; i++) {
@@ -842,7 +874,7 @@ int life_grid_destroy(life_grid* g) {
         for (void* p : {g->f[0], g->f[1], (void*)g->rho0, (void*)g->u0, (void*)g->fibm}) dfree_cached(p);
         for (void* p : {(void*)g->bc, (void*)g->u_in, (void*)g->rho_in, (void*)g->delU, (void*)g->ucol, (void*)g->epsA,
                         (void*)g->eps_first, (void*)g->eps_n, (void*)g->eps_off, (void*)g->eps_outside,
-                        (void*)g->eps_band, (void*)g->eps_y})
+                        (void*)g->eps_band, (void*)g->eps_y, (void*)g->fsite[0], (void*)g->fsite[1]})
             dfree(p);
         if (g->eps_outside_h) (void)hipHostFree(g->eps_outside_h);
         if (g->blow_h) (void)hipHostFree(g->blow_h);

@@ -70,6 +70,7 @@ struct life_grid {
     double* u0 = nullptr;
     bool explicit_state = true;
     double* fibm = nullptr;
+    double* fsite[2] = {nullptr, nullptr};  // per-site force_xy planes (WOMERSLEY with gravity)
     bool dense_clear_next = false;
     // boundaries
     std::vector<life::BCSite> bc_host;

@@ -73,7 +73,8 @@ struct Request {
 struct Slot {
     Request* req = nullptr;
     life_case* c = nullptr;
-    int moff = 0;  // first marker in the packed staging arrays
+    int moff = 0;     // first marker in the packed staging arrays
+    int fem_off = 0;  // first flexible body in the engine's device-FEM table
 };
 
 // Device footprint of a case: two population buffers, force_ibm, the
@@ -195,6 +196,15 @@ struct life_server {
     hipEvent_t ev_lattice = nullptr, ev_bulk = nullptr;
     Mirror<CaseDev> cases_bulk;     // descriptors the bulk stream-collide reads
     double early_steps = 0.0;
+    // FEM on the device (FP32-shifted populations): every case's flexible
+    // bodies back to back, and per subiteration the list of the bodies of
+    // the cases still subiterating, their step or relaxation factor and
+    // their residual terms
+    bool fem_device = false;
+    size_t fem_lds = 0;
+    Mirror<FemDevBody> fem_bodies;
+    Mirror<int> fem_list, fem_list2;
+    Mirror<double> fem_param, fem_out;
     EngineTimes times;
     // statistics
     double site_updates = 0.0, steps = 0.0, busy = 0.0, captures = 0.0, peak_running = 0.0, peak_bytes = 0.0;
@@ -237,6 +247,13 @@ void admit(life_server* s) {
         r->info.t_admit = s->now();
         r->info.bytes = bytes;
         life_case* c = nullptr;
+        if (r->p.womersley != 0.0 && (r->p.gravityX != 0.0 || r->p.gravityY != 0.0)) {
+            // force_xy per site: the batched kernels take a uniform force
+            r->info.status = LIFE_REQ_FAILED;
+            r->error = "WOMERSLEY forcing with gravity runs on the single-case path (life_case) only";
+            r->info.t_finish = s->now();
+            continue;
+        }
         if (life_case_create(&r->p, r->geometry.c_str(), s->cfg.precision, s->cfg.device, &c)) {
             r->info.status = LIFE_REQ_FAILED;
             r->error = life_last_error();
@@ -314,6 +331,32 @@ void rebuild(life_server* s) {
     s->stage_in.reserve(4 * (size_t)s->total_markers);
     s->stage_out.reserve(3 * (size_t)s->total_markers);
     s->dyn.reserve(s->slots.size());
+    // device FEM when every flexible case has it (FP32-shifted populations)
+    std::vector<FemDevBody> fb;
+    bool all_dev = true, any_flex = false;
+    s->fem_lds = 0;
+    for (Slot& sl : s->slots) {
+        life_case* c = sl.c;
+        if (!c->hasFlex) continue;
+        any_flex = true;
+        if (!c->fdev.ok()) {
+            all_dev = false;
+            continue;
+        }
+        sl.fem_off = (int)fb.size();
+        fb.insert(fb.end(), c->fdev.host.begin(), c->fdev.host.end());
+        s->fem_lds = std::max(s->fem_lds, c->fdev.lds);
+    }
+    s->fem_device = any_flex && all_dev;
+    if (s->fem_device) {
+        s->fem_bodies.reserve(fb.size());
+        std::copy(fb.begin(), fb.end(), s->fem_bodies.h);
+        s->fem_bodies.upload(fb.size(), s->stream);
+        s->fem_list.reserve(fb.size());
+        s->fem_list2.reserve(fb.size());
+        s->fem_param.reserve(fb.size());
+        s->fem_out.reserve(3 * fb.size());
+    }
     LIFE_HIP(hipStreamSynchronize(s->stream));  // host list buffers are reused next time
     drop_graph(s);
     s->dirty = false;
@@ -503,6 +546,84 @@ void for_cases(life_server* s, std::vector<int>& list, Fn&& fn) {
     list.swap(keep);
 }
 
+// The FEM half of recomputeObjectVals on the device for the active cases:
+// predictor at subiteration 0, else the Aitken relaxation (factor per case,
+// Objects.cpp:179-210).  Returns the number of listed bodies (the list
+// stays in fem_list for the solve).
+int fem_recompute_device(life_server* s, const std::vector<int>& active) {
+    int n = 0;
+    bool predict = false;
+    for (int q : active) {
+        life_case* c = s->slots[q].c;
+        predict = c->subIt == 0;  // the same for every case still subiterating
+        if (!predict) case_aitken(c);
+        for (int b = 0; b < c->fdev.nb; b++) {
+            s->fem_list.h[n] = s->slots[q].fem_off + b;
+            s->fem_param.h[n] = predict ? (double)c->t : c->relax;
+            n++;
+        }
+    }
+    s->fem_list.upload(n, s->stream);
+    s->fem_param.upload(n, s->stream);
+    if (predict)
+        launch_fem_predict(s->fem_bodies.d, s->fem_list.d, n, 0, s->fem_param.d, s->fem_lds, s->stream);
+    else
+        launch_fem_relax(s->fem_bodies.d, s->fem_list.d, n, 0.0, s->fem_param.d, s->fem_lds, s->stream);
+    LIFE_HIP(hipGetLastError());
+    return n;
+}
+
+// femKernel of the active cases on the device, then each case's
+// subiteration residual (case_fem's sums, in body order).  Epsilon systems
+// found not to be banded get the dense solve and their cases' FEM runs
+// again with it.
+void fem_solve_device(life_server* s, const std::vector<int>& active, const std::vector<CaseSys>& sysl, int nfb) {
+    hipStream_t st = s->stream;
+    launch_fem_dynamic(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_lds, false, st);
+    // the device has room beside the FEM's few workgroups: the next step's
+    // bulk stream-collide starts now
+    if (s->early == 2) launch_bulk_early(s);
+    launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.d, st);
+    LIFE_HIP(hipGetLastError());
+    s->fem_out.download(3 * (size_t)nfb, st);
+    LIFE_HIP(hipStreamSynchronize(st));
+    std::vector<int> redo;  // slots whose epsilon changed
+    for (size_t k = 0; k < sysl.size(); k++)
+        if (s->outside.h[k]) {
+            life_case* c = s->slots[sysl[k].c].c;
+            case_solve_eps_dense(c, c->g->eps_first_h[sysl[k].k], c->g->eps_n_h[sysl[k].k]);
+            LIFE_HIP(hipStreamSynchronize(c->g->stream));
+            if (redo.empty() || redo.back() != sysl[k].c) redo.push_back(sysl[k].c);
+        }
+    if (!redo.empty()) {
+        int n = 0;
+        for (int q : redo)
+            for (int b = 0; b < s->slots[q].c->fdev.nb; b++) s->fem_list2.h[n++] = s->slots[q].fem_off + b;
+        s->fem_list2.upload(n, st);
+        launch_fem_dynamic(s->fem_bodies.d, s->fem_list2.d, n, s->fem_lds, true, st);
+        launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.d, st);
+        LIFE_HIP(hipGetLastError());
+        s->fem_out.download(3 * (size_t)nfb, st);
+        LIFE_HIP(hipStreamSynchronize(st));
+    }
+    s->times.lap(st, 4);
+    s->times.subits += 1;
+    int k = 0;
+    for (int q : active) {
+        life_case* c = s->slots[q].c;
+        double res = 0.0, num = 0.0, den = 0.0;
+        for (int b = 0; b < c->fdev.nb; b++, k++) {
+            res += s->fem_out.h[3 * k];
+            num += s->fem_out.h[3 * k + 1];
+            den += s->fem_out.h[3 * k + 2];
+        }
+        c->subRes = std::sqrt(res) / (c->p.ref_L * std::sqrt((double)c->simDOFs));
+        c->subNum = num;
+        c->subDen = den;
+        c->subIt++;
+    }
+}
+
 // ObjectsClass::objectKernel (Objects.cpp:26-60) of every case with bodies.
 void coupling_phase(life_server* s) {
     std::vector<int> flex;
@@ -521,16 +642,24 @@ void coupling_phase(life_server* s) {
     std::vector<CaseSys> sysl;
     std::vector<std::pair<int, int>> dense;  // (slot, first marker of a system) needing the dense path
     std::vector<int> dense_n;
+    const bool devfem = s->fem_device;
+    int nfb = 0;  // bodies in this subiteration's device-FEM list
     while (true) {
         sysl.clear();
         dense.clear();
         dense_n.clear();
         if (!active.empty()) {
-            // recomputeObjectVals: predictor / relaxation on the host, then
-            // supports, ds and epsilon on the device
-            for_cases(s, active, [&](Slot& sl) { case_recompute_host(sl.c); });
-            s->times.lap(st, 2);
-            markers_in(s, active);
+            // recomputeObjectVals: predictor / relaxation (host or device
+            // FEM), then supports, ds and epsilon on the device
+            if (devfem) {
+                nfb = fem_recompute_device(s, active);
+                std::copy(active.begin(), active.end(), s->dyn.h);
+                s->dyn.upload(active.size(), st);
+            } else {
+                for_cases(s, active, [&](Slot& sl) { case_recompute_host(sl.c); });
+                s->times.lap(st, 2);
+                markers_in(s, active);
+            }
             const int na = (int)active.size();
             int max_n = 0;
             bool any_uni = false;
@@ -576,6 +705,15 @@ void coupling_phase(life_server* s) {
         }
         first = false;
         if (active.empty()) break;
+        if (devfem) {
+            fem_solve_device(s, active, sysl, nfb);
+            std::vector<int> next;
+            for (int q : active)
+                if (case_subiterate(s->slots[q].c)) next.push_back(q);
+            active.swap(next);
+            if (active.empty()) break;
+            continue;
+        }
         launch_markers_out_batch(s->cases.d, s->dyn.d, (int)active.size(), s->max_markers, s->stage_out.d, st);
         LIFE_HIP(hipGetLastError());
         s->stage_out.download(3 * (size_t)s->total_markers, st);
@@ -619,11 +757,13 @@ void coupling_phase(life_server* s) {
     launch_spread_batch(s->cases.d, s->lists.d + s->off_ibm_all, s->n_ibm_all, s->max_markers, st);
     LIFE_HIP(hipGetLastError());
     // the last FEM solve moved the markers: the device takes the final
-    // positions (as life_case_step does after its spread)
-    markers_in(s, flex);
+    // positions (as life_case_step does after its spread); with the FEM on
+    // the device they are there already, and the host copies lag
+    if (!devfem) markers_in(s, flex);
     for (int q : flex) {
         life_case* c = s->slots[q].c;
-        c->forces_on_host = true;
+        c->forces_on_host = !devfem;
+        c->host_stale = devfem;
         if (c->subIt == kMaxSubIt) fail(s, s->slots[q], "Subiteration scheme hit 20 iterations");
     }
 }
@@ -634,6 +774,7 @@ void retire(life_server* s, Slot& sl) {
     life_case* c = sl.c;
     if (r->info.status != LIFE_REQ_FAILED) {
         try {
+            case_sync_host(c);
             if (c->hasIBM) check_flags(c->g);
             if (life_case_total_force(c, r->info.total_force)) throw std::runtime_error(life_last_error());
             if (c->nm > 0) {
@@ -809,6 +950,11 @@ int life_server_destroy(life_server* s) {
         s->outside.release();
         s->stage_in.release();
         s->stage_out.release();
+        s->fem_bodies.release();
+        s->fem_list.release();
+        s->fem_list2.release();
+        s->fem_param.release();
+        s->fem_out.release();
         if (s->ev_lattice) (void)hipEventDestroy(s->ev_lattice);
         if (s->ev_bulk) (void)hipEventDestroy(s->ev_bulk);
         if (s->stream) (void)hipStreamDestroy(s->stream);

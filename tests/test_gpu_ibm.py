@@ -202,3 +202,63 @@ def test_layer1_epsilon_matrix(oracle, gpu_lib):
     assert rel_err(eps, oc.markers()["eps"]) <= TOL
     # epsilon * A = 1 (Objects.cpp:303-306)
     assert np.max(np.abs(A @ eps - 1.0)) < 1e-10
+
+
+@pytest.mark.parametrize("cm", [1, 0])
+def test_gravity_flexible_matches_oracle(oracle, gpu_lib, cm):
+    """Gravity: the uniform body force on the fluid (Grid.cpp:1042-1043) and
+    the weight of the beam elements in the FEM load (FEMElement.cpp:35),
+    with a flexible plate, FP64 on the GPU against the oracle; the weight
+    visibly moves the plate against the same case without gravity."""
+    p0, g = small_fsi(uni=1, cm=cm)
+    p = p0.replace(gravityX=1e-6, gravityY=-6e-6)
+    oc, gc = build(oracle, p, g)
+    steps = 20
+    oc.step(steps)
+    gc.step(steps)
+    co, cg = oc.coupling(), gc.coupling()
+    assert co["subIt"] == cg["subIt"]
+    U_o, U_g = oc.fem(1)[0], gc.fem(1)[0]
+    assert rel_err(U_g, U_o) <= TOL
+    compare_markers(oc, gc)
+    compare_fields(oc, gc)
+    assert rel_err(gc.total_force(), oc.total_force()) <= TOL
+    o0 = oracle.OracleCase(p0, g)
+    o0.step(steps)
+    dU = float(np.max(np.abs(U_o - o0.fem(1)[0])) / np.max(np.abs(U_o)))
+    print("gravity cm %d: subIt %d, FEM displacement change from gravity %.3g" % (cm, cg["subIt"], dU), flush=True)
+    assert dU > 1e-3
+    o0.close()
+    oc.close()
+    gc.close()
+
+
+def test_womersley_gravity_flexible_matches_oracle(oracle, gpu_lib):
+    """The reference's PELskin example (WOMERSLEY, ten clamped flexible
+    filaments, physical units, 735 x 91) with gravity added: force_xy per
+    site from each site's density and the weight of the beam elements in
+    the FEM load, FP64 against the oracle; the weight roughly doubles the
+    filaments' displacement over these steps."""
+    p0, g = cfg.pelskin()
+    p = p0.replace(gravityX=4.0, gravityY=-9.81)
+    oracle.set_threads(16)
+    oc, gc = build(oracle, p, g)
+    steps = 40
+    oc.step(steps)
+    gc.step(steps)
+    co, cg = oc.coupling(), gc.coupling()
+    assert co["subIt"] == cg["subIt"]
+    for b in range(oc.n_bodies):
+        assert rel_err(gc.fem(b)[0], oc.fem(b)[0]) <= TOL
+    compare_markers(oc, gc)
+    compare_fields(oc, gc)
+    assert rel_err(gc.total_force(), oc.total_force()) <= TOL
+    o0 = oracle.OracleCase(p0, g)
+    o0.step(steps)
+    change = float(np.max(np.abs(oc.fem(0)[0] - o0.fem(0)[0])) / np.max(np.abs(o0.fem(0)[0])))
+    print("PELskin with gravity: subIt %d, FEM displacement change from gravity %.3g" % (cg["subIt"], change),
+          flush=True)
+    assert change > 0.1
+    o0.close()
+    oc.close()
+    gc.close()

@@ -117,12 +117,38 @@ def test_womersley_fp32_shifted_accuracy(oracle, gpu_lib):
     assert rel_err(rg, ro) <= 1e-6
 
 
-def test_womersley_rejects_gravity(gpu_lib):
+@pytest.mark.parametrize("cm", [0, 1])
+@pytest.mark.parametrize("kind", ["periodic", "convective"])
+def test_womersley_gravity_matches_oracle(oracle, gpu_lib, kind, cm):
+    """WOMERSLEY with gravity (Grid.cpp:52-61): force_xy of every site from
+    its density at the start of the step.  Gravity across the channel
+    stratifies the density (a few per cent here), so the force differs from
+    site to site.  FP64 equals the oracle at 1e-12, from the initial state
+    and from a random state uploaded through layer 1 (whose first step
+    takes the uploaded density)."""
+    p, g = womersley_params(kind, cm)
+    p = p.replace(gravityX=1e-5, gravityY=-2e-4)
+    for state in (None, random_state(p.Nx, p.Ny, seed=7)):
+        oc, gc = run_both(oracle, p, g, 90, state=state)
+        for _ in range(2):
+            fo, ro, uo, _ = oc.grid()
+            fg, rg, ug, _ = gc.fields()
+            assert rel_err(rg, ro) <= TOL
+            assert rel_err(ug, uo) <= TOL
+            assert rel_err(fg, fo) <= TOL
+            oc.step(70)
+            gc.step(70)
+        assert float(ro.max() - ro.min()) > 1e-2
+        oc.close()
+        gc.close()
+
+
+def test_womersley_gravity_needs_fp64(gpu_lib):
     from life_amd.case import Case
     from life_amd.lib import LifeError
-    p = cfg.womersley_channel().replace(gravityX=1e-6)
-    with pytest.raises(LifeError, match="WOMERSLEY"):
-        Case(p, "", precision=0)
+    p = cfg.womersley_channel().replace(gravityY=-1e-4)
+    with pytest.raises(LifeError, match="needs FP64"):
+        Case(p, "", precision=1)
 
 
 @pytest.mark.parametrize("kind,steps,bar", [("ldc64", 5000, 3e-6), ("cylinder", 5000, 1e-6)])

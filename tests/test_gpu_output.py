@@ -44,12 +44,19 @@ def womersley():
     return p.replace(womersley=8.0), g
 
 
-@pytest.mark.parametrize("which", ["fsi", "canopy", "womersley"])
+def womersley_gravity():
+    """Per-site force_xy (WOMERSLEY with gravity): the restarted step takes
+    its force from the restart's density."""
+    p, g = womersley()
+    return p.replace(gravityY=-1e-4), g
+
+
+@pytest.mark.parametrize("which", ["fsi", "canopy", "womersley", "womersley_gravity"])
 def test_restart_continues_bitwise(tmp_path, which):
     """Run A for n1 + n2 steps; run B for n1 steps, write the restart, and
     run C from that restart for n2 steps: C equals A bit for bit (for the
     oscillating force too: it resumes at the restart's time step)."""
-    p, g = {"fsi": fsi, "canopy": canopy, "womersley": womersley}[which]()
+    p, g = {"fsi": fsi, "canopy": canopy, "womersley": womersley, "womersley_gravity": womersley_gravity}[which]()
     n1, n2 = 9, 7
     a = Case(p, g, precision=L.LIFE_FP64)
     a.step(n1 + n2)

@@ -183,3 +183,17 @@ def test_timed_stream_runs_to_completion():
     st = srv.stats()
     assert st["site_updates"] == sum(r["Nx"] * r["Ny"] * 8 for r in stream)
     srv.close()
+
+
+def test_womersley_gravity_request_fails_alone():
+    """A WOMERSLEY case with gravity (force_xy per site) is refused by the
+    engine with a message; the request beside it finishes as usual."""
+    pw, gw = cfg.womersley_cylinder()
+    srv = Server(precision=L.LIFE_FP64, max_cases=4)
+    bad = srv.submit(pw.replace(womersley=8.0, gravityY=-1e-4), gw, 5)
+    good = srv.submit(*cylinder(), 5, keep_fields=True)
+    srv.drain()
+    assert srv.info(bad)["status"] == L.LIFE_REQ_FAILED
+    assert "single-case path" in srv.error(bad)
+    assert srv.info(good)["status"] == L.LIFE_REQ_DONE, srv.error(good)
+    srv.close()
