@@ -497,20 +497,21 @@ void case_blowup_cadence(life_case* c) {
     blowup_cadence(g, c->t, g->stream);
 }
 
-// The FEM runs on the device for FP32-shifted populations when the
-// environment sets LIFE_DEVICE_FEM = 1; the plain method always keeps the
-// host FEM, whose transcendental functions match the oracle's bit for bit.
+// The FEM runs on the device for FP32-shifted populations unless the
+// environment sets LIFE_DEVICE_FEM = 0 (or a body does not fit the device
+// FEM's LDS); the plain method always keeps the host FEM, whose
+// transcendental functions match the oracle's bit for bit.
 bool device_fem_wanted(int precision) {
     if (precision != LIFE_FP32_SHIFTED) return false;
     const char* e = std::getenv("LIFE_DEVICE_FEM");
-    return e && std::atoi(e) != 0;
+    return !(e && *e && std::atoi(e) == 0);
 }
 
 void setup_device_fem(life_case* c) {
     std::vector<const FemBody*> fb;
     std::vector<int> first, nm;
     for (Body* b : c->flex_bodies) {
-        if (fem_dev_lds_bytes(b->fem->ndof()) > kFemMaxLds) return;  // too large: host FEM
+        if (fem_dev_lds_bytes(b->fem->ndof(), b->n) > kFemMaxLds) return;  // too large: host FEM
         fb.push_back(b->fem.get());
         first.push_back(b->first);
         nm.push_back(b->n);

@@ -15,10 +15,9 @@ namespace life {
 
 class FemBody;
 
-// Vectors of one body, ndof each, in this order in FemDevBody::vec.
-enum FemVec { kFU, kFUd, kFUdd, kFUn, kFUnm1, kFUnm2, kFUdn, kFUddn, kFUkm1, kFRk, kFRkm1, kFR, kFF, kFDelU, kFVec,
-              kFNVec };
-constexpr int kFemState = 11;     // U .. Rkm1: the state a restart or the host copy needs
+// Vectors of one body, ndof each, in this order in FemDevBody::vec: the
+// state a restart or the host copy needs.
+enum FemVec { kFU, kFUd, kFUdd, kFUn, kFUnm1, kFUnm2, kFUdn, kFUddn, kFUkm1, kFRk, kFRkm1, kFemState };
 constexpr int kFemElStride = 16;  // per element: L0 L angle A I E rho | ca sa | F[6] | pad
 constexpr int kFemKL = 5;         // an element couples DOFs at most 5 apart
 constexpr int kFemBandW = 3 * kFemKL + 1;  // band row: columns r-5 .. r+10 (room for row-swap fill)
@@ -32,6 +31,7 @@ struct FemDevScalars {
 struct FemDevBody {
     int ndof, nbc, nel, nnode, nm;
     FemDevScalars s;
+    // vec, node, el, Mloc and KLloc lie in this order in one device block
     double* node;          // nnode x 6: pos0 x, y, pos x, y, angle0, angle
     double* el;            // nel x kFemElStride
     const double* Mloc;    // nel x 36 (constant)
@@ -41,9 +41,7 @@ struct FemDevBody {
     const double* map_z;   // 2 per entry
     const int* pm_el;      // nm: element of each marker
     const double* pm_zeta; // nm
-    double* vec;           // kFNVec x ndof
-    double* M;             // ndof x kFemBandW (global mass, band)
-    double* K;             // ndof x kFemBandW (global stiffness, band)
+    double* vec;           // kFemState x ndof
     double* out;           // subRes, subNum, subDen, itNR, resNR
     // the body's markers (physical positions and velocities, lattice
     // forces, epsilon), already offset to its first marker
@@ -65,10 +63,13 @@ struct FemDevCase {
     bool ok() const { return nb > 0 && d_bodies != nullptr; }
 };
 
-// LDS the solve needs for a body of ndof unknowns (band, right-hand side,
-// row extents); bodies beyond the device limit stay on the host FEM.
-size_t fem_dev_lds_bytes(int ndof);
-constexpr size_t kFemMaxLds = 60 * 1024;
+// LDS a launch needs for a body of ndof unknowns and nm markers (the
+// solve's global matrices, vectors and marker loads, and the body's state
+// and element data mirrored from device memory); bodies beyond the limit
+// (gfx950: 160 KiB of LDS per workgroup, less the kernels' own) stay on the
+// host FEM.
+size_t fem_dev_lds_bytes(int ndof, int nm);
+constexpr size_t kFemMaxLds = 156 * 1024;
 
 // Device state from the host bodies (their present state).  mk: the case's
 // marker arrays; first[b], nm[b]: each body's marker range.

@@ -31,6 +31,11 @@ namespace {
 #define FDI __device__ __forceinline__
 
 constexpr int kNZd[6][2] = {{0, 1}, {0, 1}, {2, -1}, {3, 4}, {3, 4}, {5, -1}};
+// kNZd[i][q] for a row index known only at run time
+FDI int kNZdRow(int i, int q) {
+    if (i == 2 || i == 5) return q == 0 ? i : -1;
+    return q == 0 ? (i < 3 ? 0 : 3) : (i < 3 ? 1 : 4);
+}
 
 // Utils::shiftAngle (Utils.h:235-247), as wrap_angle in fem.cpp.
 FDI double wrap_angle_d(double a) {
@@ -104,6 +109,13 @@ FDI void wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
+
+// Loads through the global address space: they cannot alias LDS, so the
+// compiler may keep many in flight across LDS stores.
+typedef const __attribute__((address_space(1))) double g_double;
+typedef const __attribute__((address_space(1))) int g_int;
+FDI double gload(const double* p) { return *(g_double*)p; }
+FDI int gload(const int* p) { return *(g_int*)p; }
 
 FDI double* vecp(const FemDevBody& B, int k) { return B.vec + (size_t)k * B.ndof; }
 FDI double& band(double* A, int r, int c) { return A[(size_t)r * kFemBandW + (c - r + kFemKL)]; }
@@ -219,19 +231,44 @@ FDI void update_ibm_d(const FemDevBody& B, double* elu) {
     __syncthreads();
 }
 
+// LDS of the dynamic solve (doubles): K band (ndof x kFemBandW, with room
+// for the fill of row swaps; the elimination runs on it in place), M band
+// (ndof x kMBand), the force vector F, the load vector R, the Newmark
+// vector, the increment delU, the elimination's row extents, and the
+// body's marker forces and epsilon (3 per marker).  update_ibm's element
+// terms reuse the K band.
+constexpr int kMBand = 2 * kFemKL + 1;
+struct FemLds {
+    double *K, *M, *F, *R, *vec, *x, *mk;
+    int* last;
+};
+FDI FemLds fem_lds(const FemDevBody& B, double* lds) {
+    FemLds L;
+    const size_t n = (size_t)B.ndof;
+    L.K = lds;
+    L.M = L.K + n * kFemBandW;
+    L.F = L.M + n * kMBand;
+    L.R = L.F + n;
+    L.vec = L.R + n;
+    L.x = L.vec + n;
+    L.last = reinterpret_cast<int*>(L.x + n);
+    L.mk = L.x + 2 * n;
+    return L;
+}
+FDI double& mband(double* M, int r, int c) { return M[(size_t)r * kMBand + (c - r + kFemKL)]; }
+
 // One load-map entry of element q (FemBody::load_vector, FEMElement.cpp:27-69):
-// its global load terms RG.
-FDI void load_entry_d(const FemDevBody& B, int q, int entry, double RG[6]) {
+// its global load terms RG.  mk: the body's marker forces (x, y) and
+// epsilon, 3 per marker; k: the entry's marker, a and b: its zeta range.
+FDI void load_entry_d(const FemDevBody& B, const double* mk, int q, int k, double a, double b, double RG[6]) {
     const double* e = elp(B, q);
     const double L = e[1], A = e[3], rho = e[6];
     double T[6][6];
     make_T(e[7], e[8], T);
     const double wt0 = rho * A * B.s.gx, wt1 = rho * A * B.s.gy;
     const double fs = B.s.Dm / (B.s.Dt * B.s.Dt);
-    const int k = B.map_node[entry];
-    const double a = B.map_z[2 * entry], b = B.map_z[2 * entry + 1];
-    const double sc = -B.meps[k] * 1.0 * fs;
-    const double vv[2] = {sc * B.mforce[2 * k] + wt0, sc * B.mforce[2 * k + 1] + wt1};
+    const double sc = -mk[3 * k + 2] * 1.0 * fs;
+    const double vv[2] = {sc * mk[3 * k] + wt0, sc * mk[3 * k + 1] + wt1};
     double Fl[2];
 #pragma unroll
     for (int r = 0; r < 2; r++) Fl[r] = 0.0 + T[r][0] * vv[0] + T[r][1] * vv[1];
@@ -248,249 +285,362 @@ FDI void load_entry_d(const FemDevBody& B, int q, int entry, double RG[6]) {
     mul_t6_d(T, R, RG);
 }
 
-// R (FEMBodyClass::constructRVector): node n's DOFs take the terms of
-// element n-1 (its second node) then of element n (its first node), each
-// in load-map order, onto 0.0.
-FDI void load_vector_d(const FemDevBody& B) {
-    double* R = vecp(B, kFR);
-    for (int n = threadIdx.x; n < B.nnode; n += blockDim.x) {
+// The load-map entries en0 .. en1-1 of element q, added in map order onto
+// acc (the node's terms of the element: its first node's RG[0..2] or its
+// second node's RG[3..5]).  The entries' map values are loaded eight at a
+// time, ahead of the arithmetic.
+FDI void add_entries_d(const FemDevBody& B, const double* mk, int q, int en0, int en1, int half, double acc[3]) {
+    constexpr int U = 8;
+    for (int base = en0; base < en1; base += U) {
+        int kk[U];
+        double za[U], zb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int en = base + u;
+            kk[u] = en < en1 ? gload(B.map_node + en) : 0;
+            za[u] = en < en1 ? gload(B.map_z + 2 * en) : 0.0;
+            zb[u] = en < en1 ? gload(B.map_z + 2 * en + 1) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (base + u < en1) {
+                double RG[6];
+                load_entry_d(B, mk, q, kk[u], za[u], zb[u], RG);
+                acc[0] += RG[3 * half];
+                acc[1] += RG[3 * half + 1];
+                acc[2] += RG[3 * half + 2];
+            }
+        }
+    }
+}
+
+// R (FEMBodyClass::constructRVector) into LDS: node n's DOFs take the terms
+// of element n-1 (its second node) then of element n (its first node),
+// each in load-map order, onto 0.0.
+FDI void load_vector_d(const FemDevBody& B, const FemLds& S) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int k = tid; k < B.nm; k += nt) {
+        S.mk[3 * k] = gload(B.mforce + 2 * k);
+        S.mk[3 * k + 1] = gload(B.mforce + 2 * k + 1);
+        S.mk[3 * k + 2] = gload(B.meps + k);
+    }
+    __syncthreads();
+    for (int n = tid; n < B.nnode; n += nt) {
         double acc[3] = {0.0, 0.0, 0.0};
-        if (n >= 1) {
-            const int q = n - 1;
-            for (int en = B.map_start[q]; en < B.map_start[q + 1]; en++) {
-                double RG[6];
-                load_entry_d(B, q, en, RG);
-                acc[0] += RG[3];
-                acc[1] += RG[4];
-                acc[2] += RG[5];
-            }
-        }
-        if (n < B.nel) {
-            for (int en = B.map_start[n]; en < B.map_start[n + 1]; en++) {
-                double RG[6];
-                load_entry_d(B, n, en, RG);
-                acc[0] += RG[0];
-                acc[1] += RG[1];
-                acc[2] += RG[2];
-            }
-        }
-        R[3 * n] = acc[0];
-        R[3 * n + 1] = acc[1];
-        R[3 * n + 2] = acc[2];
+        const int s0 = n >= 1 ? gload(B.map_start + n - 1) : 0;
+        const int s1 = gload(B.map_start + n);
+        const int s2 = n < B.nel ? gload(B.map_start + n + 1) : 0;
+        if (n >= 1) add_entries_d(B, S.mk, n - 1, s0, s1, 1, acc);
+        if (n < B.nel) add_entries_d(B, S.mk, n, s1, s2, 0, acc);
+        S.R[3 * n] = acc[0];
+        S.R[3 * n + 1] = acc[1];
+        S.R[3 * n + 2] = acc[2];
     }
     __syncthreads();
 }
 
-// force_vector + mass_stiff of element q (FEMElement.cpp:72-138): the
-// element's terms added onto the global F, M and K.
-FDI void element_terms_d(const FemDevBody& B, int q) {
-    double* e = B.el + (size_t)q * kFemElStride;
-    const double L0 = e[0], L = e[1], angle = e[2], A = e[3], I = e[4], E = e[5];
+// Entry (r, c) of the element's rotation matrix make_T(ca, sa).
+FDI double t_entry(double ca, double sa, int r, int c) {
+    if (r / 3 != c / 3) return 0.0;
+    const int a = r % 3, b = c % 3;
+    if (a == 2 || b == 2) return a == b ? 1.0 : 0.0;
+    return a == b ? ca : (a == 0 ? sa : -sa);
+}
+
+// Row i of Transpose(T) A T, with the terms of rotate6_d in its order:
+// the rows of A that row i of Transpose(T) A reads are a0 and a1 (a1
+// unused when row i reads one row), weighted by T[ra][i] and T[rb][i].
+FDI void rotate_row_d(double ca, double sa, int i, const double a0[6], const double a1[6], double out[6]) {
+    const int ra = kNZdRow(i, 0), rb = kNZdRow(i, 1);
+    const double t0 = t_entry(ca, sa, ra, i), t1 = rb >= 0 ? t_entry(ca, sa, rb, i) : 0.0;
+    double tmp[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double s = 0.0;
+        s += t0 * a0[j];
+        if (rb >= 0) s += t1 * a1[j];
+        tmp[j] = s;
+    }
     double T[6][6];
-    make_T(e[7], e[8], T);
-    const double* nd0 = B.node + 6 * q;
-    const double* nd1 = B.node + 6 * (q + 1);
+    make_T(ca, sa, T);
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (kNZd[j][q] >= 0) s += tmp[kNZd[j][q]] * T[kNZd[j][q]][j];
+        out[j] = s;
+    }
+}
+
+// Row r of the element's geometric stiffness K_NL (FEMElement.cpp:119-131).
+FDI void knl_row(int r, double Fa, double V0, double L0, double out[6]) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) out[j] = 0.0;
+    switch (r) {
+        case 0: out[1] = -V0 / L0; out[4] = V0 / L0; break;
+        case 1: out[0] = -V0 / L0; out[1] = Fa / L0; out[3] = V0 / L0; out[4] = -Fa / L0; break;
+        case 3: out[1] = V0 / L0; out[4] = -V0 / L0; break;
+        case 4: out[0] = V0 / L0; out[1] = -Fa / L0; out[3] = -V0 / L0; out[4] = Fa / L0; break;
+        default: break;
+    }
+}
+
+// force_vector + mass_stiff of element q (FEMElement.cpp:72-138), row i of
+// the element's terms (one lane per row): added onto the global F, M and K
+// in LDS.  Each entry takes the same operations as the whole-element form.
+// Every value the row reads from device memory is loaded first, so the
+// loads share one round trip.
+FDI void element_row_d(const FemDevBody& B, const FemLds& S, int q, int i) {
+    const int ra = kNZdRow(i, 0), rb = kNZdRow(i, 1);
+    const int rb1 = rb >= 0 ? rb : ra;
+    const double* e = B.el + (size_t)q * kFemElStride;
+    const double* Ml = B.Mloc + 36 * (size_t)q;
+    const double* Kl = B.KLloc + 36 * (size_t)q;
+    double ev[9], m0[6], m1[6], k0[6], k1[6];
+#pragma unroll
+    for (int k = 0; k < 9; k++) ev[k] = e[k];
+    const double nth0 = B.node[6 * q + 5], nth1 = B.node[6 * (q + 1) + 5];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        m0[j] = Ml[6 * ra + j];
+        m1[j] = Ml[6 * rb1 + j];
+        k0[j] = Kl[6 * ra + j];
+        k1[j] = Kl[6 * rb1 + j];
+    }
+    const double L0 = ev[0], L = ev[1], angle = ev[2], A = ev[3], I = ev[4], E = ev[5];
+    const double ca = ev[7], sa = ev[8];
     const double u = (L * L - L0 * L0) / (L + L0);
-    const double th1 = wrap_angle_d(nd0[5] - angle);
-    const double th2 = wrap_angle_d(nd1[5] - angle);
+    const double th1 = wrap_angle_d(nth0 - angle);
+    const double th2 = wrap_angle_d(nth1 - angle);
     const double F0 = (E * A / L0) * u;
     const double M1 = (2 * E * I / L0) * (2.0 * th1 + th2);
     const double M2 = (2 * E * I / L0) * (th1 + 2.0 * th2);
-    double Fe[6];
-    Fe[0] = -F0;
-    Fe[1] = (1.0 / L0) * (M1 + M2);
-    Fe[2] = M1;
-    Fe[3] = F0;
-    Fe[4] = -(1.0 / L0) * (M1 + M2);
-    Fe[5] = M2;
-    double FG[6];
-    mul_t6_d(T, Fe, FG);
-    const int d0 = 3 * q;
-    double* Fg = vecp(B, kFF);
+    // Fe (the element's local forces): the two entries row i reads
+    const double Fsum = (1.0 / L0) * (M1 + M2);
+    auto fe = [&](int r) {
+        switch (r) {
+            case 0: return -F0;
+            case 1: return Fsum;
+            case 2: return M1;
+            case 3: return F0;
+            case 4: return -((1.0 / L0) * (M1 + M2));
+            default: return M2;
+        }
+    };
+    // row i of Transpose(T) Fe (mul_t6_d)
+    double fg = 0.0;
+    fg += t_entry(ca, sa, ra, i) * fe(ra);
+    if (rb >= 0) fg += t_entry(ca, sa, rb, i) * fe(rb);
+    const int d0 = 3 * q, row = d0 + i;
+    double G[6], N[6], n0[6], n1[6];
+    rotate_row_d(ca, sa, i, m0, m1, G);
+    S.F[row] += fg;
 #pragma unroll
-    for (int k = 0; k < 6; k++) Fg[d0 + k] += FG[k];
-    double Al[6][6], G[6][6], N[6][6];
-    const double* Ml = B.Mloc + 36 * (size_t)q;
+    for (int j = 0; j < 6; j++) mband(S.M, row, d0 + j) += G[j];
+    rotate_row_d(ca, sa, i, k0, k1, G);
+    const double Fa = F0, V0 = -((1.0 / L0) * (M1 + M2));
+    knl_row(ra, Fa, V0, L0, n0);
+    knl_row(rb1, Fa, V0, L0, n1);
+    rotate_row_d(ca, sa, i, n0, n1, N);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-#pragma unroll
-        for (int j = 0; j < 6; j++) Al[i][j] = Ml[6 * i + j];
-    rotate6_d(T, Al, G);
-#pragma unroll
-    for (int i = 0; i < 6; i++)
-#pragma unroll
-        for (int j = 0; j < 6; j++) band(B.M, d0 + i, d0 + j) += G[i][j];
-    const double* Kl = B.KLloc + 36 * (size_t)q;
-#pragma unroll
-    for (int i = 0; i < 6; i++)
-#pragma unroll
-        for (int j = 0; j < 6; j++) Al[i][j] = Kl[6 * i + j];
-    rotate6_d(T, Al, G);
-    const double Fa = -Fe[0], V0 = Fe[4];
-    double KNL[6][6];
-#pragma unroll
-    for (int i = 0; i < 6; i++)
-#pragma unroll
-        for (int j = 0; j < 6; j++) KNL[i][j] = 0.0;
-    KNL[0][1] = -V0 / L0;
-    KNL[0][4] = V0 / L0;
-    KNL[1][0] = -V0 / L0;
-    KNL[1][1] = Fa / L0;
-    KNL[1][3] = V0 / L0;
-    KNL[1][4] = -Fa / L0;
-    KNL[3][1] = V0 / L0;
-    KNL[3][4] = -V0 / L0;
-    KNL[4][0] = V0 / L0;
-    KNL[4][1] = -Fa / L0;
-    KNL[4][3] = -V0 / L0;
-    KNL[4][4] = Fa / L0;
-    rotate6_d(T, KNL, N);
-#pragma unroll
-    for (int i = 0; i < 6; i++)
-#pragma unroll
-        for (int j = 0; j < 6; j++) band(B.K, d0 + i, d0 + j) += G[i][j] + N[i][j];
+    for (int j = 0; j < 6; j++) band(S.K, row, d0 + j) += G[j] + N[j];
 }
 
-// FemBody::build_global (FEMBody.cpp:90-109)
-FDI void build_global_d(const FemDevBody& B) {
+// FemBody::build_global (FEMBody.cpp:90-109) into LDS.  Elements q and q+1
+// share a node, so the even elements go first, then the odd ones; an entry
+// takes at most two terms, whose sum does not depend on their order.
+FDI void build_global_d(const FemDevBody& B, const FemLds& S) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nb = B.ndof * kFemBandW;
-    for (int k = tid; k < nb; k += nt) {
-        B.M[k] = 0.0;
-        B.K[k] = 0.0;
-    }
-    double* F = vecp(B, kFF);
-    for (int k = tid; k < B.ndof; k += nt) F[k] = 0.0;
+    for (int k = tid; k < B.ndof * kFemBandW; k += nt) S.K[k] = 0.0;
+    for (int k = tid; k < B.ndof * kMBand; k += nt) S.M[k] = 0.0;
+    for (int k = tid; k < B.ndof; k += nt) S.F[k] = 0.0;
     __syncthreads();
+    const int nph = (B.nel + 1) / 2;  // elements of one parity, at most
     for (int ph = 0; ph < 2; ph++) {
-        for (int q = ph + 2 * tid; q < B.nel; q += 2 * nt) element_terms_d(B, q);
+        for (int t = tid; t < 6 * nph; t += nt) {
+            const int q = ph + 2 * (t / 6);
+            if (q < B.nel) element_row_d(B, S, q, t % 6);
+        }
         __syncthreads();
     }
 }
 
-// lu_solve_band (lu.cpp) on wavefront 0: K x = F over the trailing block
-// (rows/columns >= nbc), x into delU.  LDS: W (m x kFemBandW), y (m),
-// last (m ints).
-FDI void lu_solve_d(const FemDevBody& B, double* W, double* y, int* last) {
-    const int L = threadIdx.x;
-    const int n = B.ndof, bc = B.nbc, m = n - bc, kl = kFemKL, ku = kFemKL, kext = kl + ku;
-    const double* Kg = B.K;
-    const double* b = vecp(B, kFF);
-    double* x = vecp(B, kFDelU);
-    if (L < 64) {
-        for (int r = L; r < m; r += 64) {
-            const int lo = r - kl < 0 ? 0 : r - kl;
-            const int hi = r + ku > m - 1 ? m - 1 : r + ku;
-            const int top = r + kext > m - 1 ? m - 1 : r + kext;
-            for (int c = lo; c <= hi; c++)
-                W[(size_t)r * kFemBandW + (c - r + kl)] = Kg[(size_t)(r + bc) * kFemBandW + (c - r + kl)];
-            for (int c = hi + 1; c <= top; c++) W[(size_t)r * kFemBandW + (c - r + kl)] = 0.0;
-            last[r] = hi;
-            y[r] = b[r + bc];
-        }
-        wave_fence();
-        for (int k = 0; k < m; k++) {
-            const int ilast = k + kl > m - 1 ? m - 1 : k + kl;
-            int p = k;
-            double best = fabs(W[(size_t)k * kFemBandW + kl]);
-            for (int i = k + 1; i <= ilast; i++) {
-                const double a = fabs(W[(size_t)i * kFemBandW + (k - i + kl)]);
-                if (a > best) {
-                    best = a;
-                    p = i;
-                }
-            }
-            if (best == 0.0) continue;
-            if (p != k) {
-                const int jl = last[k] > last[p] ? last[k] : last[p];
-                for (int j = k + L; j <= jl; j += 64) {
-                    double& a = W[(size_t)k * kFemBandW + (j - k + kl)];
-                    double& c = W[(size_t)p * kFemBandW + (j - p + kl)];
-                    const double t = a;
-                    a = c;
-                    c = t;
-                }
-                wave_fence();
-                if (L == 0) {
-                    const int tl = last[k];
-                    last[k] = last[p];
-                    last[p] = tl;
-                    const double t = y[k];
-                    y[k] = y[p];
-                    y[p] = t;
-                }
-                wave_fence();
-            }
-            const double piv = W[(size_t)k * kFemBandW + kl];
-            const int jlast = last[k];
-            const int nr = ilast - k;
-            if (L >= 1 && L <= nr) {
-                double& a = W[(size_t)(k + L) * kFemBandW + (k - (k + L) + kl)];
-                a = a / piv;
-            }
-            wave_fence();
-            const int ncol = jlast - k;
-            for (int idx = L; idx < nr * ncol; idx += 64) {
-                const int r = 1 + idx / ncol, c = k + 1 + idx % ncol;
-                const int i = k + r;
-                const double l = W[(size_t)i * kFemBandW + (k - i + kl)];
-                double& a = W[(size_t)i * kFemBandW + (c - i + kl)];
-                a = a - l * W[(size_t)k * kFemBandW + (c - k + kl)];
-            }
-            if (L >= 1 && L <= nr) {
-                const int i = k + L;
-                const double l = W[(size_t)i * kFemBandW + (k - i + kl)];
-                y[i] = y[i] - l * y[k];
-                if (last[i] < jlast) last[i] = jlast;
-            }
-            wave_fence();
-        }
-        if (L == 0) {
-            for (int i = m - 1; i >= 0; i--) {
-                const double* ri = W + (size_t)i * kFemBandW + kl - i;
-                double s = y[i];
-                const int jlast = last[i];
-                for (int j = i + 1; j <= jlast; j++) s = s - ri[j] * x[j + bc];
-                x[i + bc] = s / ri[i];
-            }
-            for (int i = 0; i < bc; i++) x[i] = 0.0;
+// Inverse of a 3x3 block (row-major, in place) by its adjugate.
+FDI void inv3_d(double* D) {
+    const double a = D[0], b = D[1], c = D[2], d = D[3], e = D[4], f = D[5], g = D[6], h = D[7], i = D[8];
+    const double c00 = e * i - f * h, c01 = f * g - d * i, c02 = d * h - e * g;
+    const double det = a * c00 + b * c01 + c * c02;
+    const double r = 1.0 / det;
+    D[0] = c00 * r;
+    D[1] = (c * h - b * i) * r;
+    D[2] = (b * f - c * e) * r;
+    D[3] = c01 * r;
+    D[4] = (a * i - c * g) * r;
+    D[5] = (c * d - a * f) * r;
+    D[6] = c02 * r;
+    D[7] = (b * g - a * h) * r;
+    D[8] = (a * e - b * d) * r;
+}
+
+// Node blocks of the cyclic reduction: sub-diagonal A, diagonal D (its
+// inverse once the node is eliminated), super-diagonal C, right-hand side f.
+constexpr int kBlkA = 0, kBlkD = 9, kBlkC = 18, kBlkF = 27, kBlk = 30;
+
+// K x = F (K + a0 M from newton_d, band in LDS) with x = 0 on the nbc
+// clamped DOFs.  An element couples two neighbouring nodes, so K is block
+// tridiagonal in 3 x 3 node blocks: A_n x_{n-1} + D_n x_n + C_n x_{n+1} =
+// f_n.  The clamped DOFs become identity rows and columns, which leaves the
+// other unknowns' equations as lu_solve_band's trailing block.  Block cyclic
+// reduction: at stride s = 1, 2, 4, ... the nodes at odd multiples of s are
+// eliminated from their neighbours' equations (the nodes at multiples of
+// 2s), so the chain of dependent steps is log2(nnode) levels long instead
+// of nnode; back substitution then runs the levels in reverse.  K is
+// symmetric positive definite (the Newmark mass term dominates), so no
+// pivoting is needed.  The values agree with the host's scalar elimination
+// to rounding, not bit for bit.  Blocks: kBlk doubles per node in the M
+// band's space (free once K + a0 M is formed); a level's new blocks are
+// staged in the K band's space (free once the blocks are read out of it).
+FDI void block_solve_d(const FemDevBody& B, const FemLds& S) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int N = B.nnode, nbc = B.nbc;
+    // constrained DOFs: identity rows and columns, zero right-hand side
+    for (int t = tid; t < nbc * (2 * kFemKL + 1); t += nt) {
+        const int d = t / (2 * kFemKL + 1), o = t % (2 * kFemKL + 1) - kFemKL;  // o = column - row
+        const int c = d + o;
+        if (c >= 0 && c < B.ndof) {
+            band(S.K, d, c) = (o == 0) ? 1.0 : 0.0;  // row d
+            if (o != 0) band(S.K, c, d) = 0.0;        // column d
         }
     }
+    for (int d = tid; d < nbc; d += nt) S.F[d] = 0.0;
+    __syncthreads();
+    double* blk = S.M;
+    for (int t = tid; t < N * kBlk; t += nt) {
+        const int n = t / kBlk, q = t % kBlk;
+        double v;
+        if (q < kBlkF) {
+            const int cn = n + q / 9 - 1, i = (q % 9) / 3, j = q % 3;  // column node n-1, n, n+1
+            v = (cn < 0 || cn >= N) ? 0.0 : band(S.K, 3 * n + i, 3 * cn + j);
+        } else {
+            v = S.F[3 * n + q - kBlkF];
+        }
+        blk[t] = v;
+    }
+    __syncthreads();
+    double* stage = S.K;
+    int s = 1;
+    for (; s < N; s *= 2) {
+        // nodes j = s, 3s, 5s, ... leave: D_j becomes its inverse
+        for (int j = s + 2 * s * tid; j < N; j += 2 * s * nt) inv3_d(blk + kBlk * j + kBlkD);
+        __syncthreads();
+        // nodes i = 0, 2s, 4s, ... take them out, entry (r, c) per lane:
+        //   al = A_i inv(D_{i-s}), ga = C_i inv(D_{i+s})
+        //   D_i - al C_{i-s} - ga A_{i+s},  -al A_{i-s},  -ga C_{i+s},
+        //   f_i - al f_{i-s} - ga f_{i+s}
+        const int nr = (N - 1) / (2 * s) + 1;
+        for (int t = tid; t < 9 * nr; t += nt) {
+            const int k = t / 9, r = (t % 9) / 3, c = t % 3, i = 2 * s * k;
+            const double* bi = blk + kBlk * i;
+            double Dn = bi[kBlkD + 3 * r + c], An = 0.0, Cn = 0.0, fn = bi[kBlkF + r];
+            if (i - s >= 0) {
+                const double* bj = blk + kBlk * (i - s);
+                double al[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    al[q] = bi[kBlkA + 3 * r] * bj[kBlkD + q] + bi[kBlkA + 3 * r + 1] * bj[kBlkD + 3 + q] +
+                            bi[kBlkA + 3 * r + 2] * bj[kBlkD + 6 + q];
+                Dn = Dn - (al[0] * bj[kBlkC + c] + al[1] * bj[kBlkC + 3 + c] + al[2] * bj[kBlkC + 6 + c]);
+                An = -(al[0] * bj[kBlkA + c] + al[1] * bj[kBlkA + 3 + c] + al[2] * bj[kBlkA + 6 + c]);
+                fn = fn - (al[0] * bj[kBlkF] + al[1] * bj[kBlkF + 1] + al[2] * bj[kBlkF + 2]);
+            }
+            if (i + s < N) {
+                const double* bj = blk + kBlk * (i + s);
+                double ga[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    ga[q] = bi[kBlkC + 3 * r] * bj[kBlkD + q] + bi[kBlkC + 3 * r + 1] * bj[kBlkD + 3 + q] +
+                            bi[kBlkC + 3 * r + 2] * bj[kBlkD + 6 + q];
+                Dn = Dn - (ga[0] * bj[kBlkA + c] + ga[1] * bj[kBlkA + 3 + c] + ga[2] * bj[kBlkA + 6 + c]);
+                Cn = -(ga[0] * bj[kBlkC + c] + ga[1] * bj[kBlkC + 3 + c] + ga[2] * bj[kBlkC + 6 + c]);
+                fn = fn - (ga[0] * bj[kBlkF] + ga[1] * bj[kBlkF + 1] + ga[2] * bj[kBlkF + 2]);
+            }
+            double* o = stage + kBlk * k;
+            o[kBlkA + 3 * r + c] = An;
+            o[kBlkD + 3 * r + c] = Dn;
+            o[kBlkC + 3 * r + c] = Cn;
+            if (c == 0) o[kBlkF + r] = fn;
+        }
+        __syncthreads();
+        for (int t = tid; t < kBlk * nr; t += nt) blk[kBlk * 2 * s * (t / kBlk) + t % kBlk] = stage[t];
+        __syncthreads();
+    }
+    // node 0 alone: x_0 = inv(D_0) f_0
+    if (tid == 0) inv3_d(blk + kBlkD);
+    __syncthreads();
+    if (tid < 3) {
+        const double* b0 = blk;
+        S.x[tid] = b0[kBlkD + 3 * tid] * b0[kBlkF] + b0[kBlkD + 3 * tid + 1] * b0[kBlkF + 1] +
+                   b0[kBlkD + 3 * tid + 2] * b0[kBlkF + 2];
+    }
+    __syncthreads();
+    // back substitution, levels in reverse: x_j = inv(D_j) (f_j - A_j x_{j-s} - C_j x_{j+s})
+    for (s /= 2; s >= 1; s /= 2) {
+        const int ne = (N + s - 1) / (2 * s);
+        for (int t = tid; t < 3 * ne; t += nt) {
+            const int j = s + 2 * s * (t / 3), r = t % 3;
+            const double* bj = blk + kBlk * j;
+            const double* xl = S.x + 3 * (j - s);
+            double v[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                v[a] = bj[kBlkF + a] - (bj[kBlkA + 3 * a] * xl[0] + bj[kBlkA + 3 * a + 1] * xl[1] +
+                                        bj[kBlkA + 3 * a + 2] * xl[2]);
+            if (j + s < N) {
+                const double* xr = S.x + 3 * (j + s);
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    v[a] = v[a] - (bj[kBlkC + 3 * a] * xr[0] + bj[kBlkC + 3 * a + 1] * xr[1] +
+                                   bj[kBlkC + 3 * a + 2] * xr[2]);
+            }
+            S.x[3 * j + r] = bj[kBlkD + 3 * r] * v[0] + bj[kBlkD + 3 * r + 1] * v[1] + bj[kBlkD + 3 * r + 2] * v[2];
+        }
+        __syncthreads();
+    }
+    for (int d = tid; d < nbc; d += nt) S.x[d] = 0.0;
     __syncthreads();
 }
 
 // FemBody::newton_dynamic (FEMBody.cpp:71-87, 112-126) and the residual
 // test of dynamic(): returns resNR (every lane).
-FDI double newton_d(const FemDevBody& B, double* lds) {
+FDI double newton_d(const FemDevBody& B, const FemLds& S) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    build_global_d(B);
+    build_global_d(B, S);
     const double Dt = B.s.Dt, al = B.s.alpha;
     const double a0 = 1.0 / (al * (Dt * Dt));
     const double a2 = 1.0 / (al * Dt);
     const double a3 = 1.0 / (2.0 * al) - 1.0;
     const int nd = B.ndof;
     double *U = vecp(B, kFU), *Un = vecp(B, kFUn), *Ud = vecp(B, kFUd), *Udd = vecp(B, kFUdd);
-    double *vec = vecp(B, kFVec), *F = vecp(B, kFF), *R = vecp(B, kFR), *delU = vecp(B, kFDelU);
-    for (int k = tid; k < nd; k += nt) vec[k] = a0 * (Un[k] - U[k]) + a2 * Ud[k] + a3 * Udd[k];
+    for (int k = tid; k < nd; k += nt) S.vec[k] = a0 * (Un[k] - U[k]) + a2 * Ud[k] + a3 * Udd[k];
     __syncthreads();
     const int kb = kFemKL;
     for (int i = tid; i < nd; i += nt) {
         double s = 0.0;
         const int lo = i - kb < 0 ? 0 : i - kb, hi = i + kb > nd - 1 ? nd - 1 : i + kb;
-        for (int j = lo; j <= hi; j++) s += band(B.M, i, j) * vec[j];
-        F[i] = R[i] - F[i] + s;
-        for (int j = lo; j <= hi; j++) band(B.K, i, j) = band(B.K, i, j) + a0 * band(B.M, i, j);
+        for (int j = lo; j <= hi; j++) s += mband(S.M, i, j) * S.vec[j];
+        S.F[i] = S.R[i] - S.F[i] + s;
+        for (int j = lo; j <= hi; j++) band(S.K, i, j) = band(S.K, i, j) + a0 * mband(S.M, i, j);
     }
     __syncthreads();
-    const int m = nd - B.nbc;
-    lu_solve_d(B, lds, lds + (size_t)m * kFemBandW, reinterpret_cast<int*>(lds + (size_t)m * (kFemBandW + 1)));
-    for (int k = tid; k < nd; k += nt) U[k] = U[k] + delU[k];
+    block_solve_d(B, S);
+    for (int k = tid; k < nd; k += nt) U[k] = U[k] + S.x[k];
     __syncthreads();
     update_fem_d(B);
     __shared__ double res_sh;
     if (tid == 0) {
         double r = 0.0;
-        for (int k = 0; k < nd; k++) r += delU[k] * delU[k];
+        for (int k = 0; k < nd; k++) r += S.x[k] * S.x[k];
         res_sh = sqrt(r) / (B.s.ref_L * sqrt((double)nd));
     }
     __syncthreads();
@@ -503,6 +653,49 @@ FDI void copy_vec(const FemDevBody& B, int to, int from) {
     for (int k = threadIdx.x; k < B.ndof; k += blockDim.x) d[k] = s[k];
 }
 
+// LDS of a launch: the solve's scratch (fem_lds), then the body mirrored
+// from device memory: its state vectors U .. Rkm1, nodes, elements and
+// element matrices.  The kernels run on the mirror (one round trip to
+// device memory in, one out) and write the markers and out[] directly.
+__host__ __device__ inline size_t scratch_doubles(int ndof, int nm) { return (size_t)ndof * (kFemBandW + kMBand + 5) + 3 * (size_t)nm; }
+
+FDI FemDevBody mirror_in(const FemDevBody& G, double* lds, bool mats) {
+    FemDevBody B = G;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nv = kFemState * G.ndof, nn = 6 * G.nnode, ne = kFemElStride * G.nel, nk = mats ? 36 * G.nel : 0;
+    double* m = lds + scratch_doubles(G.ndof, G.nm);
+    B.vec = m;
+    B.node = m + nv;
+    B.el = B.node + nn;
+    B.Mloc = B.el + ne;
+    B.KLloc = B.Mloc + nk;
+    // vec, node, el, Mloc, KLloc lie in this order in one device block:
+    // one contiguous copy, sixteen loads in flight per lane
+    const int total = nv + nn + ne + 2 * nk;
+    constexpr int U = 16;
+    for (int base = 0; base < total; base += U * nt) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = gload(G.vec + min(base + u * nt + tid, total - 1));
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = base + u * nt + tid;
+            if (k < total) m[k] = v[u];
+        }
+    }
+    __syncthreads();
+    return B;
+}
+
+FDI void mirror_out(const FemDevBody& G, const FemDevBody& B) {
+    __syncthreads();
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nv = kFemState * G.ndof, nn = 6 * G.nnode, ne = kFemElStride * G.nel;
+    for (int k = tid; k < nv; k += nt) G.vec[k] = B.vec[k];
+    for (int k = tid; k < nn; k += nt) G.node[k] = B.node[k];
+    for (int k = tid; k < ne; k += nt) G.el[k] = B.el[k];
+}
+
 // list: the bodies of this launch (null: 0 .. nb-1); param: per listed
 // body, its step t (predict) or relaxation factor (relax); null: value.
 FDI int body_index(const int* list) { return list ? list[blockIdx.x] : (int)blockIdx.x; }
@@ -511,7 +704,8 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_predict(const FemDevBody* _
                                                             const int* __restrict__ list, int t_all,
                                                             const double* __restrict__ t_d) {
     extern __shared__ double lds[];
-    const FemDevBody B = bodies[body_index(list)];
+    const FemDevBody G = bodies[body_index(list)];
+    const FemDevBody B = mirror_in(G, lds, false);
     const int t = t_d ? (int)t_d[blockIdx.x] : t_all;
     // reset_values (FEMBody.cpp:341-349): the history moves back one step
     copy_vec(B, kFUnm2, kFUnm1);
@@ -534,6 +728,7 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_predict(const FemDevBody* _
     finish_newmark_d(B);
     update_ibm_d(B, lds);
     copy_vec(B, kFUkm1, kFU);
+    mirror_out(G, B);
 }
 
 // ObjectsClass::recomputeObjectVals relaxation (Objects.cpp:198-210)
@@ -541,7 +736,8 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_relax(const FemDevBody* __r
                                                           const int* __restrict__ list, double omega,
                                                           const double* __restrict__ omega_d) {
     extern __shared__ double lds[];
-    const FemDevBody B = bodies[body_index(list)];
+    const FemDevBody G = bodies[body_index(list)];
+    const FemDevBody B = mirror_in(G, lds, false);
     const double w = omega_d ? omega_d[blockIdx.x] : omega;
     double *U = vecp(B, kFU), *Ukm1 = vecp(B, kFUkm1);
     for (int k = threadIdx.x; k < B.ndof; k += blockDim.x) U[k] = Ukm1[k] + w * (U[k] - Ukm1[k]);
@@ -550,6 +746,7 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_relax(const FemDevBody* __r
     finish_newmark_d(B);
     update_ibm_d(B, lds);
     copy_vec(B, kFUkm1, kFU);
+    mirror_out(G, B);
 }
 
 // FemBody::dynamic (FEMBody.cpp:26-68) and subResidual (FEMBody.cpp:244-256)
@@ -558,17 +755,19 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_relax(const FemDevBody* __r
 __global__ __launch_bounds__(kFemThreads) void k_fem_dynamic(const FemDevBody* __restrict__ bodies,
                                                             const int* __restrict__ list, int redo) {
     extern __shared__ double lds[];
-    const FemDevBody B = bodies[body_index(list)];
+    const FemDevBody G = bodies[body_index(list)];
+    const FemDevBody B = mirror_in(G, lds, true);
     copy_vec(B, kFU, kFUn);
     copy_vec(B, kFUd, kFUdn);
     copy_vec(B, kFUdd, kFUddn);
     __syncthreads();
     update_fem_d(B);
-    load_vector_d(B);
+    const FemLds S = fem_lds(B, lds);
+    load_vector_d(B, S);
     int it = 0;
     double res;
     do {
-        res = newton_d(B, lds);
+        res = newton_d(B, S);
         it++;
     } while (res > 1e-10 && it < 20);
     finish_newmark_d(B);
@@ -577,19 +776,30 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_dynamic(const FemDevBody* _
     if (!redo) copy_vec(B, kFRkm1, kFRk);
     __syncthreads();
     double *Rk = vecp(B, kFRk), *Rkm1 = vecp(B, kFRkm1), *U = vecp(B, kFU), *Ukm1 = vecp(B, kFUkm1);
-    for (int k = threadIdx.x; k < B.ndof; k += blockDim.x) Rk[k] = U[k] - Ukm1[k];
+    // Rk and Rkm1 also into LDS (F and R are free now) for the sums below
+    for (int k = threadIdx.x; k < B.ndof; k += blockDim.x) {
+        const double rk = U[k] - Ukm1[k];
+        Rk[k] = rk;
+        S.F[k] = rk;
+        S.R[k] = Rkm1[k];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
+        // the three sums in DOF order each, side by side
         double r = 0.0, num = 0.0, den = 0.0;
-        for (int k = 0; k < B.ndof; k++) r += Rk[k] * Rk[k];
-        for (int k = 0; k < B.ndof; k++) num += Rkm1[k] * (Rk[k] - Rkm1[k]);
-        for (int k = 0; k < B.ndof; k++) den += (Rk[k] - Rkm1[k]) * (Rk[k] - Rkm1[k]);
+        for (int k = 0; k < B.ndof; k++) {
+            const double rk = S.F[k], rm = S.R[k];
+            r += rk * rk;
+            num += rm * (rk - rm);
+            den += (rk - rm) * (rk - rm);
+        }
         B.out[0] = r;
         B.out[1] = num;
         B.out[2] = den;
         B.out[3] = (double)it;
         B.out[4] = res;
     }
+    mirror_out(G, B);
 }
 
 // out[0..2] of every listed body, packed (3 per body)
@@ -603,40 +813,60 @@ __global__ void k_fem_outs(const FemDevBody* __restrict__ bodies, const int* __r
     outs[3 * q + 2] = o[2];
 }
 
-__global__ void k_fem_sums(const FemDevBody* __restrict__ bodies, int nb, double* __restrict__ sums) {
-    double r = 0.0, num = 0.0, den = 0.0;
-    for (int b = 0; b < nb; b++) {
+// Sums over the bodies in body order (case_fem): the terms are gathered in
+// parallel, then added by one lane.
+__global__ __launch_bounds__(256) void k_fem_sums(const FemDevBody* __restrict__ bodies, int nb,
+                                                 double* __restrict__ sums) {
+    __shared__ double t[3 * 256];
+    for (int b = threadIdx.x; b < nb && b < 256; b += blockDim.x) {
         const double* o = bodies[b].out;
-        r += o[0];
-        num += o[1];
-        den += o[2];
+        t[3 * b] = o[0];
+        t[3 * b + 1] = o[1];
+        t[3 * b + 2] = o[2];
     }
-    sums[0] = r;
-    sums[1] = num;
-    sums[2] = den;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0, num = 0.0, den = 0.0;
+        for (int b = 0; b < nb; b++) {
+            const double* o = b < 256 ? t + 3 * b : bodies[b].out;
+            r += o[0];
+            num += o[1];
+            den += o[2];
+        }
+        sums[0] = r;
+        sums[1] = num;
+        sums[2] = den;
+    }
 }
 
 }  // namespace
 
-size_t fem_dev_lds_bytes(int ndof) {
-    // LU: band + right-hand side + extents; update_ibm: 12 per element
-    const size_t lu = sizeof(double) * (size_t)ndof * (kFemBandW + 2);
-    const size_t ibm = sizeof(double) * 12 * (size_t)(ndof / 3);
-    return lu > ibm ? lu : ibm;
+size_t fem_dev_lds_bytes(int ndof, int nm) {
+    // the dynamic solve's scratch (fem_lds; update_ibm's 12 per element fit
+    // in its K band), then the mirror of a chain of ndof / 3 nodes
+    const size_t nnode = (size_t)ndof / 3, nel = nnode - 1;
+    const size_t mirror = kFemState * (size_t)ndof + 6 * nnode + (kFemElStride + 72) * nel;
+    return sizeof(double) * (scratch_doubles(ndof, nm) + mirror);
 }
 
 void launch_fem_predict(const FemDevBody* b, const int* list, int nb, int t, const double* t_d, size_t lds,
                         hipStream_t st) {
-    if (nb > 0) k_fem_predict<<<nb, kFemThreads, lds, st>>>(b, list, t, t_d);
+    if (nb == 0) return;
+    k_fem_predict<<<nb, kFemThreads, lds, st>>>(b, list, t, t_d);
+    LIFE_HIP(hipGetLastError());
 }
 
 void launch_fem_relax(const FemDevBody* b, const int* list, int nb, double omega, const double* omega_d, size_t lds,
                       hipStream_t st) {
-    if (nb > 0) k_fem_relax<<<nb, kFemThreads, lds, st>>>(b, list, omega, omega_d);
+    if (nb == 0) return;
+    k_fem_relax<<<nb, kFemThreads, lds, st>>>(b, list, omega, omega_d);
+    LIFE_HIP(hipGetLastError());
 }
 
 void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st) {
-    if (nb > 0) k_fem_dynamic<<<nb, kFemThreads, lds, st>>>(b, list, redo ? 1 : 0);
+    if (nb == 0) return;
+    k_fem_dynamic<<<nb, kFemThreads, lds, st>>>(b, list, redo ? 1 : 0);
+    LIFE_HIP(hipGetLastError());
 }
 
 void launch_fem_outs(const FemDevBody* b, const int* list, int nb, double* outs, hipStream_t st) {
@@ -644,7 +874,7 @@ void launch_fem_outs(const FemDevBody* b, const int* list, int nb, double* outs,
 }
 
 void launch_fem_sums(const FemDevBody* b, int nb, double* sums, hipStream_t st) {
-    if (nb > 0) k_fem_sums<<<1, 1, 0, st>>>(b, nb, sums);
+    if (nb > 0) k_fem_sums<<<1, 256, 0, st>>>(b, nb, sums);
 }
 
 // ---------------------------------------------------------------- host side
@@ -662,7 +892,7 @@ T* upload_vec(FemDevCase& fc, const std::vector<T>& v, hipStream_t st) {
 // state (the element's ca/sa follow from its angle).
 void pack_state(const FemBody& fb, const FemBody::Layout& lay, std::vector<double>& vecs, std::vector<double>& el) {
     const int nd = fb.ndof();
-    vecs.assign((size_t)kFNVec * nd, 0.0);
+    vecs.assign((size_t)kFemState * nd, 0.0);
     const std::vector<double>* v[] = {&fb.U, &fb.Ud, &fb.Udd, &fb.Un, &fb.Unm1, &fb.Unm2,
                                       &fb.Udn, &fb.Uddn, &fb.Ukm1, &fb.Rk, &fb.Rkm1};
     for (int k = 0; k < kFemState; k++) std::memcpy(&vecs[(size_t)k * nd], v[k]->data(), sizeof(double) * nd);
@@ -697,30 +927,43 @@ void fem_dev_create(FemDevCase& fc, const std::vector<const FemBody*>& bodies, c
         d.nnode = fb.num_nodes();
         d.nm = nm[b];
         d.s = s;
-        d.node = upload_vec(fc, lay.node, st);
-        d.el = upload_vec(fc, el, st);
-        d.Mloc = upload_vec(fc, lay.Mloc, st);
-        d.KLloc = upload_vec(fc, lay.KLloc, st);
+        // the block solve takes a chain of 3 x 3 node blocks
+        if (d.ndof != 3 * d.nnode || d.nel != d.nnode - 1 || d.nbc > d.ndof || d.nbc < 0)
+            throw std::runtime_error("device FEM: a body must be a chain of nodes with 3 DOFs each");
+        if (lay.node.size() != (size_t)6 * d.nnode || lay.Mloc.size() != (size_t)36 * d.nel ||
+            lay.KLloc.size() != (size_t)36 * d.nel)
+            throw std::runtime_error("device FEM: unexpected body layout");
+        // one block: vectors, nodes, elements, element matrices (the kernels
+        // copy its leading part into LDS in one run)
+        std::vector<double> slab(vecs);
+        slab.insert(slab.end(), lay.node.begin(), lay.node.end());
+        slab.insert(slab.end(), el.begin(), el.end());
+        slab.insert(slab.end(), lay.Mloc.begin(), lay.Mloc.end());
+        slab.insert(slab.end(), lay.KLloc.begin(), lay.KLloc.end());
+        d.vec = upload_vec(fc, slab, st);
+        d.node = d.vec + (size_t)kFemState * d.ndof;
+        d.el = d.node + (size_t)6 * d.nnode;
+        d.Mloc = d.el + (size_t)kFemElStride * d.nel;
+        d.KLloc = d.Mloc + (size_t)36 * d.nel;
         d.map_start = upload_vec(fc, lay.map_start, st);
         d.map_node = upload_vec(fc, lay.map_node, st);
         d.map_z = upload_vec(fc, lay.map_z, st);
         d.pm_el = upload_vec(fc, lay.pm_el, st);
         d.pm_zeta = upload_vec(fc, lay.pm_zeta, st);
-        d.vec = upload_vec(fc, vecs, st);
-        std::vector<double> zeros((size_t)d.ndof * kFemBandW, 0.0);
-        d.M = upload_vec(fc, zeros, st);
-        d.K = upload_vec(fc, zeros, st);
         d.out = upload_vec(fc, std::vector<double>(8, 0.0), st);
         d.mpos = pos + 2 * (size_t)first[b];
         d.mvel = vel + 2 * (size_t)first[b];
         d.mforce = force + 2 * (size_t)first[b];
         d.meps = eps + first[b];
         fc.host.push_back(d);
-        const size_t need = fem_dev_lds_bytes(d.ndof);
+        const size_t need = fem_dev_lds_bytes(d.ndof, d.nm);
         if (need > lds) lds = need;
     }
     if (lds > kFemMaxLds) throw std::runtime_error("flexible body too large for the device FEM");
     fc.lds = lds;
+    // dynamic LDS beyond 64 KiB per workgroup
+    for (const void* k : {(const void*)k_fem_predict, (const void*)k_fem_relax, (const void*)k_fem_dynamic})
+        LIFE_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFemMaxLds));
     fc.d_bodies = upload_vec(fc, fc.host, st);
     fc.sums = dalloc<double>(4);
     fc.blocks.push_back(fc.sums);

@@ -1,13 +1,17 @@
-"""The FEM on the device (fem_dev.hip), used with FP32-shifted populations:
-the same case stepped with the device FEM and with the host FEM
+"""The FEM on the device (fem_dev.hip), the default with FP32-shifted
+populations: the same case stepped with the device FEM and with the host FEM
 (LIFE_DEVICE_FEM=0) gives the same trajectory.  The device FEM keeps the
-host's operations and order; only the device's atan2/sin/cos may round
-differently in the last bit, so the bar is 1e-9 relative after tens of
-coupled steps (the FP32 populations themselves sit ~1e-6 from FP64), with
-identical subiteration counts at every step.  Also: a restart written with
-the device FEM continues bit for bit, and the outputs that read the FEM
-state (FEM displacements, total force, restart files) see the device's
-state."""
+host's operations and order except for the linear solve of each Newton
+iteration (block cyclic reduction over the beam's 3 x 3 node blocks in
+place of the banded LU: the same solution to rounding) and the device's
+atan2/sin/cos, which may round differently in the last bit, so the bar is
+1e-9 relative after tens of coupled steps (the FP32 populations themselves
+sit ~1e-6 from FP64), with identical subiteration counts at every step.
+The cases cover both root conditions (CLAMPED: 3 constrained DOFs,
+SUPPORTED: 2) and chains of 3 to 51 nodes, so the reduction meets node
+counts of 2^k + 1, odd and even.  Also: a restart written with the device
+FEM continues bit for bit, and the outputs that read the FEM state (FEM
+displacements, total force, restart files) see the device's state."""
 import numpy as np
 import pytest
 
@@ -42,7 +46,17 @@ def long_filament():
     return p.replace(ux0_p=0.05), g
 
 
-CASES = {"fsi": (fsi, 30), "canopy": (canopy, 30), "flag": (flag, 30), "long_filament": (long_filament, 12)}
+def supported():
+    """Filaments on pinned roots (SUPPORTED) of 16 and 2 elements."""
+    p, g = cfg.canopy(Nx=320, Ny=96, n_filaments=2, L=40.0, n_el=16, min_spacing=60.0, x_range=(60, 220),
+                      ramp_steps=0.0, u_in=0.05, kb_range=(5e-2, 1e-1))
+    lines = g.replace("CLAMPED", "SUPPORTED").splitlines()
+    lines[1] = lines[1].replace("FLEXIBLE 16", "FLEXIBLE 2")
+    return p.replace(ux0_p=0.05), "\n".join(lines) + "\n"
+
+
+CASES = {"fsi": (fsi, 30), "canopy": (canopy, 30), "flag": (flag, 30), "long_filament": (long_filament, 12),
+         "supported": (supported, 30)}
 
 
 def run(pg, steps, monkeypatch, device):
@@ -74,6 +88,23 @@ def test_device_fem_matches_host_fem(monkeypatch, gpu_lib, name):
         assert e <= BAR, (k, e)
 
 
+def test_device_fem_is_default(monkeypatch, gpu_lib):
+    """Without LIFE_DEVICE_FEM set, FP32-shifted cases use the device FEM:
+    the flag case, whose device and host FEM trajectories differ in the last
+    bits after 30 steps, follows the device FEM's bit for bit."""
+    pg = flag()
+    monkeypatch.delenv("LIFE_DEVICE_FEM", raising=False)
+    p, g = pg
+    c = Case(p, g, precision=L.LIFE_FP32_SHIFTED)
+    c.step(30)
+    u_default = c.fields()[2]
+    c.close()
+    d = run(pg, 30, monkeypatch, True)
+    h = run(pg, 30, monkeypatch, False)
+    assert not np.array_equal(d["u"], h["u"])  # the two are distinguishable here
+    assert np.array_equal(u_default, d["u"])
+
+
 def test_device_fem_restart(monkeypatch, gpu_lib, tmp_path):
     """The device FEM state goes into the restart files and comes back: a
     case read from them holds the writer's FEM state and markers exactly,
@@ -100,3 +131,30 @@ def test_device_fem_restart(monkeypatch, gpu_lib, tmp_path):
     (ud, pd, Ud, sd), (uh, ph, Uh, sh) = out[True], out[False]
     assert sd == sh
     assert rel_err(ud, uh) <= BAR and rel_err(pd, ph) <= BAR and rel_err(Ud, Uh) <= BAR
+
+
+def test_served_device_fem_equals_case(monkeypatch, gpu_lib):
+    """The serving engine's batched device FEM (every flexible body of every
+    case in one launch, per-case Aitken factors) gives each case exactly its
+    single-case device-FEM result, beside cases of other sizes and kinds."""
+    from life_amd.serve import Server
+    monkeypatch.setenv("LIFE_DEVICE_FEM", "1")
+    cases = [fsi(), canopy(), flag(),
+             cfg.cylinder_channel(Nx=160, Ny=64, D=16.0, centre=(40.0, 31.5), u_max=0.06, Re=40.0, ramp_steps=0.0)]
+    steps = 14
+    ref = []
+    for p, g in cases:
+        c = Case(p, g, precision=L.LIFE_FP32_SHIFTED)
+        c.step(steps)
+        ref.append((c.fields()[1], c.fields()[2], c.markers()["pos"], c.total_force()))
+        c.close()
+    srv = Server(precision=L.LIFE_FP32_SHIFTED, max_cases=8)
+    ids = [srv.submit(p, g, steps, keep_fields=True) for p, g in cases]
+    srv.drain()
+    for rid, (rho, u, pos, force) in zip(ids, ref):
+        info = srv.info(rid)
+        assert info["status"] == L.LIFE_REQ_DONE, srv.error(rid)
+        r2, u2, p2 = srv.fields(rid)
+        assert np.array_equal(r2, rho) and np.array_equal(u2, u) and np.array_equal(p2, pos)
+        assert np.array_equal(np.array(info["total_force"]), force)
+    srv.close()
