@@ -20,7 +20,8 @@ class FemBody;
 enum FemVec { kFU, kFUd, kFUdd, kFUn, kFUnm1, kFUnm2, kFUdn, kFUddn, kFUkm1, kFRk, kFRkm1, kFemState };
 constexpr int kFemElStride = 16;  // per element: L0 L angle A I E rho | ca sa | F[6] | pad
 constexpr int kFemKL = 5;         // an element couples DOFs at most 5 apart
-constexpr int kFemBandW = 3 * kFemKL + 1;  // band row: columns r-5 .. r+10 (room for row-swap fill)
+constexpr int kFemBandW = 2 * kFemKL + 1;  // band row: columns r-5 .. r+5
+constexpr int kFemMatNZ = 13;  // upper-triangle non-zeros of an element's mass or linear stiffness matrix
 constexpr int kFemThreads = 256;
 
 struct FemDevScalars {
@@ -31,11 +32,10 @@ struct FemDevScalars {
 struct FemDevBody {
     int ndof, nbc, nel, nnode, nm;
     FemDevScalars s;
-    // vec, node, el, Mloc and KLloc lie in this order in one device block
+    // vec, node, el and mats lie in this order in one device block
     double* node;          // nnode x 6: pos0 x, y, pos x, y, angle0, angle
     double* el;            // nel x kFemElStride
-    const double* Mloc;    // nel x 36 (constant)
-    const double* KLloc;   // nel x 36 (constant)
+    const double* mats;    // nel x 2 x kFemMatNZ: M's then KL's upper-triangle non-zeros (constant)
     const int* map_start;  // nel + 1
     const int* map_node;   // marker (body-local) of each load-map entry
     const double* map_z;   // 2 per entry
@@ -57,8 +57,8 @@ struct FemDevCase {
     std::vector<FemDevBody> host;  // descriptors (device pointers)
     FemDevBody* d_bodies = nullptr;
     std::vector<void*> blocks;     // device allocations
-    double* sums = nullptr;        // device: res, num, den over the bodies, in order
-    double* sums_h = nullptr;      // page-locked copy
+    int* done = nullptr;           // device: bodies finished in the running launch
+    double* sums_h = nullptr;      // page-locked: res, num, den over the bodies, in order
     size_t lds = 0;                // dynamic LDS of the solve kernels
     bool ok() const { return nb > 0 && d_bodies != nullptr; }
 };
@@ -66,10 +66,10 @@ struct FemDevCase {
 // LDS a launch needs for a body of ndof unknowns and nm markers (the
 // solve's global matrices, vectors and marker loads, and the body's state
 // and element data mirrored from device memory); bodies beyond the limit
-// (gfx950: 160 KiB of LDS per workgroup, less the kernels' own) stay on the
-// host FEM.
+// (gfx950: 160 KiB of LDS per workgroup, less the kernels' static 6 KiB)
+// stay on the host FEM.
 size_t fem_dev_lds_bytes(int ndof, int nm);
-constexpr size_t kFemMaxLds = 156 * 1024;
+constexpr size_t kFemMaxLds = 150 * 1024;
 
 // Device state from the host bodies (their present state).  mk: the case's
 // marker arrays; first[b], nm[b]: each body's marker range.
@@ -92,13 +92,27 @@ void launch_fem_predict(const FemDevBody* b, const int* list, int nb, int t, con
 // relax (later subiterations): factor omega, or per listed body omega_d[k].
 void launch_fem_relax(const FemDevBody* b, const int* list, int nb, double omega, const double* omega_d, size_t lds,
                       hipStream_t st);
+// Where the dynamic launch over bodies 0 .. nb-1 of one case leaves the
+// case's subiteration sums (done null: nowhere).  done: a device int, zero
+// between launches; sums_h: page-locked, 3 doubles; flags (device) ->
+// flags_h (page-locked): nflags ints copied alongside.
+struct FemSumsOut {
+    int* done = nullptr;
+    double* sums_h = nullptr;
+    const int* flags = nullptr;
+    int* flags_h = nullptr;
+    int nflags = 0;
+};
+
 // femKernel: the dynamic Newmark/Newton solve of every listed body, its new
 // marker positions and velocities, and its subiteration residual terms.
-// redo: again after epsilon changed, with the previous residual kept.
-void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st);
+// redo: again after epsilon changed, with the previous residual kept.  With
+// so.done set (bodies 0 .. nb-1 only), the last workgroup also writes the
+// sums over the bodies and the flags to the host (synchronise the stream
+// before reading them).
+void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st,
+                        const FemSumsOut& so = FemSumsOut());
 // out[0..2] of the listed bodies packed into outs (3 per body).
 void launch_fem_outs(const FemDevBody* b, const int* list, int nb, double* outs, hipStream_t st);
-// out[0..2] of bodies [0, nb) summed in order into sums[0..2].
-void launch_fem_sums(const FemDevBody* b, int nb, double* sums, hipStream_t st);
 
 }  // namespace life

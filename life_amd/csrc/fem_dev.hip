@@ -119,6 +119,29 @@ FDI int gload(const int* p) { return *(g_int*)p; }
 
 FDI double* vecp(const FemDevBody& B, int k) { return B.vec + (size_t)k * B.ndof; }
 FDI double& band(double* A, int r, int c) { return A[(size_t)r * kFemBandW + (c - r + kFemKL)]; }
+
+// Position of entry (r, c) of an element's mass or linear stiffness matrix
+// among its upper-triangle non-zeros (the layout of FemDevBody::mats), -1
+// for a structural zero; both matrices are symmetric with the same pattern.
+// Branch-free: 4 bits per position lo * 6 + hi of the upper triangle in
+// three words, 15 for a zero.
+constexpr int kMatNZPos[kFemMatNZ] = {0, 3, 7, 8, 10, 11, 14, 16, 17, 21, 28, 29, 35};
+constexpr unsigned long long mat_nz_word(int w) {
+    unsigned long long v = ~0ull;
+    for (int z = 0; z < kFemMatNZ; z++)
+        if (kMatNZPos[z] / 16 == w) {
+            const int sh = 4 * (kMatNZPos[z] % 16);
+            v = (v & ~(15ull << sh)) | ((unsigned long long)z << sh);
+        }
+    return v;
+}
+__host__ __device__ inline int mat_nz(int r, int c) {
+    constexpr unsigned long long w0 = mat_nz_word(0), w1 = mat_nz_word(1), w2 = mat_nz_word(2);
+    const int lo = r < c ? r : c, hi = r < c ? c : r, p = lo * 6 + hi;
+    const unsigned long long w = p < 16 ? w0 : (p < 32 ? w1 : w2);
+    const int z = (int)((w >> (4 * (p & 15))) & 15ull);
+    return z == 15 ? -1 : z;
+}
 FDI const double* elp(const FemDevBody& B, int e) { return B.el + (size_t)e * kFemElStride; }
 
 // FemBody::update_fem (FEMBody.cpp:198-223)
@@ -231,8 +254,7 @@ FDI void update_ibm_d(const FemDevBody& B, double* elu) {
     __syncthreads();
 }
 
-// LDS of the dynamic solve (doubles): K band (ndof x kFemBandW, with room
-// for the fill of row swaps; the elimination runs on it in place), M band
+// LDS of the dynamic solve (doubles): K band (ndof x kFemBandW), M band
 // (ndof x kMBand), the force vector F, the load vector R, the Newmark
 // vector, the increment delU, the elimination's row extents, and the
 // body's marker forces and epsilon (3 per marker).  update_ibm's element
@@ -395,18 +417,19 @@ FDI void element_row_d(const FemDevBody& B, const FemLds& S, int q, int i) {
     const int ra = kNZdRow(i, 0), rb = kNZdRow(i, 1);
     const int rb1 = rb >= 0 ? rb : ra;
     const double* e = B.el + (size_t)q * kFemElStride;
-    const double* Ml = B.Mloc + 36 * (size_t)q;
-    const double* Kl = B.KLloc + 36 * (size_t)q;
+    const double* Mc = B.mats + 2 * kFemMatNZ * (size_t)q;
+    const double* Kc = Mc + kFemMatNZ;
     double ev[9], m0[6], m1[6], k0[6], k1[6];
 #pragma unroll
     for (int k = 0; k < 9; k++) ev[k] = e[k];
     const double nth0 = B.node[6 * q + 5], nth1 = B.node[6 * (q + 1) + 5];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
-        m0[j] = Ml[6 * ra + j];
-        m1[j] = Ml[6 * rb1 + j];
-        k0[j] = Kl[6 * ra + j];
-        k1[j] = Kl[6 * rb1 + j];
+        const int za = mat_nz(ra, j), zb = mat_nz(rb1, j);
+        m0[j] = za >= 0 ? Mc[za] : 0.0;
+        m1[j] = zb >= 0 ? Mc[zb] : 0.0;
+        k0[j] = za >= 0 ? Kc[za] : 0.0;
+        k1[j] = zb >= 0 ? Kc[zb] : 0.0;
     }
     const double L0 = ev[0], L = ev[1], angle = ev[2], A = ev[3], I = ev[4], E = ev[5];
     const double ca = ev[7], sa = ev[8];
@@ -662,16 +685,16 @@ __host__ __device__ inline size_t scratch_doubles(int ndof, int nm) { return (si
 FDI FemDevBody mirror_in(const FemDevBody& G, double* lds, bool mats) {
     FemDevBody B = G;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nv = kFemState * G.ndof, nn = 6 * G.nnode, ne = kFemElStride * G.nel, nk = mats ? 36 * G.nel : 0;
+    const int nv = kFemState * G.ndof, nn = 6 * G.nnode, ne = kFemElStride * G.nel,
+              nk = mats ? 2 * kFemMatNZ * G.nel : 0;
     double* m = lds + scratch_doubles(G.ndof, G.nm);
     B.vec = m;
     B.node = m + nv;
     B.el = B.node + nn;
-    B.Mloc = B.el + ne;
-    B.KLloc = B.Mloc + nk;
-    // vec, node, el, Mloc, KLloc lie in this order in one device block:
+    B.mats = B.el + ne;
+    // vec, node, el, mats lie in this order in one device block:
     // one contiguous copy, sixteen loads in flight per lane
-    const int total = nv + nn + ne + 2 * nk;
+    const int total = nv + nn + ne + nk;
     constexpr int U = 16;
     for (int base = 0; base < total; base += U * nt) {
         double v[U];
@@ -749,11 +772,52 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_relax(const FemDevBody* __r
     mirror_out(G, B);
 }
 
+// The workgroup that finishes last (an atomic count of finished bodies,
+// with device-scope fences on both sides; nothing waits) adds the bodies'
+// subiteration terms out[0..2] in body order, as case_fem does, and writes
+// them and the epsilon systems' out-of-band flags into page-locked host
+// memory, then re-arms the count for the next launch.
+FDI void sums_by_last(const FemDevBody* __restrict__ bodies, const FemSumsOut& so) {
+    __shared__ int last;
+    __shared__ double t[3 * kFemThreads];
+    __syncthreads();  // every lane's writes of this body precede the count
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(so.done, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int nb = gridDim.x;
+    for (int b = threadIdx.x; b < nb && b < kFemThreads; b += blockDim.x) {
+        const double* o = bodies[b].out;
+        t[3 * b] = o[0];
+        t[3 * b + 1] = o[1];
+        t[3 * b + 2] = o[2];
+    }
+    for (int k = threadIdx.x; k < so.nflags; k += blockDim.x) so.flags_h[k] = so.flags[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0, num = 0.0, den = 0.0;
+        for (int b = 0; b < nb; b++) {
+            const double* o = b < kFemThreads ? t + 3 * b : bodies[b].out;
+            r += o[0];
+            num += o[1];
+            den += o[2];
+        }
+        so.sums_h[0] = r;
+        so.sums_h[1] = num;
+        so.sums_h[2] = den;
+        atomicExch(so.done, 0);
+    }
+}
+
 // FemBody::dynamic (FEMBody.cpp:26-68) and subResidual (FEMBody.cpp:244-256)
 // redo: the same solve again after epsilon changed (the dense fallback of
 // computeEpsilon): Rkm1 already holds the previous subiteration's Rk.
 __global__ __launch_bounds__(kFemThreads) void k_fem_dynamic(const FemDevBody* __restrict__ bodies,
-                                                            const int* __restrict__ list, int redo) {
+                                                            const int* __restrict__ list, int redo,
+                                                            FemSumsOut so) {
     extern __shared__ double lds[];
     const FemDevBody G = bodies[body_index(list)];
     const FemDevBody B = mirror_in(G, lds, true);
@@ -800,6 +864,7 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_dynamic(const FemDevBody* _
         B.out[4] = res;
     }
     mirror_out(G, B);
+    if (so.done) sums_by_last(bodies, so);
 }
 
 // out[0..2] of every listed body, packed (3 per body)
@@ -813,39 +878,13 @@ __global__ void k_fem_outs(const FemDevBody* __restrict__ bodies, const int* __r
     outs[3 * q + 2] = o[2];
 }
 
-// Sums over the bodies in body order (case_fem): the terms are gathered in
-// parallel, then added by one lane.
-__global__ __launch_bounds__(256) void k_fem_sums(const FemDevBody* __restrict__ bodies, int nb,
-                                                 double* __restrict__ sums) {
-    __shared__ double t[3 * 256];
-    for (int b = threadIdx.x; b < nb && b < 256; b += blockDim.x) {
-        const double* o = bodies[b].out;
-        t[3 * b] = o[0];
-        t[3 * b + 1] = o[1];
-        t[3 * b + 2] = o[2];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0, num = 0.0, den = 0.0;
-        for (int b = 0; b < nb; b++) {
-            const double* o = b < 256 ? t + 3 * b : bodies[b].out;
-            r += o[0];
-            num += o[1];
-            den += o[2];
-        }
-        sums[0] = r;
-        sums[1] = num;
-        sums[2] = den;
-    }
-}
-
 }  // namespace
 
 size_t fem_dev_lds_bytes(int ndof, int nm) {
     // the dynamic solve's scratch (fem_lds; update_ibm's 12 per element fit
     // in its K band), then the mirror of a chain of ndof / 3 nodes
     const size_t nnode = (size_t)ndof / 3, nel = nnode - 1;
-    const size_t mirror = kFemState * (size_t)ndof + 6 * nnode + (kFemElStride + 72) * nel;
+    const size_t mirror = kFemState * (size_t)ndof + 6 * nnode + (kFemElStride + 2 * kFemMatNZ) * nel;
     return sizeof(double) * (scratch_doubles(ndof, nm) + mirror);
 }
 
@@ -863,9 +902,11 @@ void launch_fem_relax(const FemDevBody* b, const int* list, int nb, double omega
     LIFE_HIP(hipGetLastError());
 }
 
-void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st) {
+void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st,
+                        const FemSumsOut& so) {
     if (nb == 0) return;
-    k_fem_dynamic<<<nb, kFemThreads, lds, st>>>(b, list, redo ? 1 : 0);
+    if (so.done && list) throw std::logic_error("device FEM: sums of a listed launch");
+    k_fem_dynamic<<<nb, kFemThreads, lds, st>>>(b, list, redo ? 1 : 0, so);
     LIFE_HIP(hipGetLastError());
 }
 
@@ -873,9 +914,6 @@ void launch_fem_outs(const FemDevBody* b, const int* list, int nb, double* outs,
     if (nb > 0) k_fem_outs<<<(nb + 63) / 64, 64, 0, st>>>(b, list, nb, outs);
 }
 
-void launch_fem_sums(const FemDevBody* b, int nb, double* sums, hipStream_t st) {
-    if (nb > 0) k_fem_sums<<<1, 256, 0, st>>>(b, nb, sums);
-}
 
 // ---------------------------------------------------------------- host side
 
@@ -934,17 +972,28 @@ void fem_dev_create(FemDevCase& fc, const std::vector<const FemBody*>& bodies, c
             lay.KLloc.size() != (size_t)36 * d.nel)
             throw std::runtime_error("device FEM: unexpected body layout");
         // one block: vectors, nodes, elements, element matrices (the kernels
-        // copy its leading part into LDS in one run)
+        // copy it into LDS in one run); of M and KL only the upper-triangle
+        // non-zeros, the host's values exactly
         std::vector<double> slab(vecs);
         slab.insert(slab.end(), lay.node.begin(), lay.node.end());
         slab.insert(slab.end(), el.begin(), el.end());
-        slab.insert(slab.end(), lay.Mloc.begin(), lay.Mloc.end());
-        slab.insert(slab.end(), lay.KLloc.begin(), lay.KLloc.end());
+        for (int q = 0; q < d.nel; q++)
+            for (const std::vector<double>* m : {&lay.Mloc, &lay.KLloc}) {
+                double nz[kFemMatNZ];
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) {
+                        const double v = (*m)[(size_t)36 * q + 6 * i + j];
+                        const int z = mat_nz(i, j);
+                        if (z < 0 ? v != 0.0 : (i <= j ? false : v != nz[z]))
+                            throw std::runtime_error("device FEM: element matrix outside the beam pattern");
+                        if (z >= 0 && i <= j) nz[z] = v;
+                    }
+                slab.insert(slab.end(), nz, nz + kFemMatNZ);
+            }
         d.vec = upload_vec(fc, slab, st);
         d.node = d.vec + (size_t)kFemState * d.ndof;
         d.el = d.node + (size_t)6 * d.nnode;
-        d.Mloc = d.el + (size_t)kFemElStride * d.nel;
-        d.KLloc = d.Mloc + (size_t)36 * d.nel;
+        d.mats = d.el + (size_t)kFemElStride * d.nel;
         d.map_start = upload_vec(fc, lay.map_start, st);
         d.map_node = upload_vec(fc, lay.map_node, st);
         d.map_z = upload_vec(fc, lay.map_z, st);
@@ -965,8 +1014,9 @@ void fem_dev_create(FemDevCase& fc, const std::vector<const FemBody*>& bodies, c
     for (const void* k : {(const void*)k_fem_predict, (const void*)k_fem_relax, (const void*)k_fem_dynamic})
         LIFE_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFemMaxLds));
     fc.d_bodies = upload_vec(fc, fc.host, st);
-    fc.sums = dalloc<double>(4);
-    fc.blocks.push_back(fc.sums);
+    fc.done = dalloc<int>(1);
+    fc.blocks.push_back(fc.done);
+    LIFE_HIP(hipMemsetAsync(fc.done, 0, sizeof(int), st));
     LIFE_HIP(hipHostMalloc((void**)&fc.sums_h, 4 * sizeof(double), hipHostMallocDefault));
     LIFE_HIP(hipStreamSynchronize(st));
 }
@@ -976,7 +1026,7 @@ void fem_dev_destroy(FemDevCase& fc) {
     fc.blocks.clear();
     fc.host.clear();
     fc.d_bodies = nullptr;
-    fc.sums = nullptr;
+    fc.done = nullptr;
     if (fc.sums_h) (void)hipHostFree(fc.sums_h);
     fc.sums_h = nullptr;
     fc.nb = 0;

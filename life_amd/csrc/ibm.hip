@@ -8,6 +8,8 @@
 
 #include <cstdlib>
 
+#include <stdexcept>
+
 #include "d2q9_f64.h"
 #include "launch.h"
 
@@ -91,17 +93,39 @@ __device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g,
         for (int t = threadIdx.x; t < 2 * cnt; t += blockDim.x) spos[t] = m.pos[2 * first + t];
     __syncthreads();
     const double* P = lds ? spos : m.pos + 2 * first;
+    const double2* P2 = reinterpret_cast<const double2*>(P);
     // sqrt(.)/Dx is monotone, so the minimum of sqrt(d.d)/Dx over the other
-    // markers is sqrt(min d.d)/Dx: one square root per marker, same value
+    // markers is sqrt(min d.d)/Dx: one square root per marker, same value.
+    // The minimum of a set does not depend on the order of the comparisons,
+    // so eight running minima take eight markers per round, their positions
+    // loaded (16 bytes each) before the comparisons.
+    constexpr int U = 8;
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
         const double x = P[2 * k], y = P[2 * k + 1];
-        double d2min = INFINITY;
-        for (int q = 0; q < cnt; q++) {
-            if (q == k) continue;
+        double mn[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) mn[u] = INFINITY;
+        int q0 = 0;
+        for (; q0 + U <= cnt; q0 += U) {
+            double2 pq[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) pq[u] = P2[q0 + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const double dx = x - pq[u].x, dy = y - pq[u].y;
+                const double d2 = 0.0 + dx * dx + dy * dy;
+                if (q0 + u != k && d2 < mn[u]) mn[u] = d2;
+            }
+        }
+        for (int q = q0; q < cnt; q++) {
             const double dx = x - P[2 * q], dy = y - P[2 * q + 1];
             const double d2 = 0.0 + dx * dx + dy * dy;
-            if (d2 < d2min) d2min = d2;
+            if (q != k && d2 < mn[0]) mn[0] = d2;
         }
+        double d2min = mn[0];
+#pragma unroll
+        for (int u = 1; u < U; u++)
+            if (mn[u] < d2min) d2min = mn[u];
         const double mag = sqrt(d2min) / g.Dx;
         m.ds[first + k] = mag < 10.0 ? mag : 10.0;
     }
@@ -196,12 +220,59 @@ __device__ inline void eps_far_pair(const MarkerDev& m, const GridScalars& g, in
         *outside = 1;
 }
 
+// The out-of-band check of one system (the pairs of eps_far_pair) by one
+// workgroup, which writes the system's flag (1: a pair outside the band
+// couples, 0: none).  Positions are staged in LDS (P: 2n doubles, or null to
+// read them from device memory).  Eight lanes share a row r and take every
+// eighth column beyond the band, four columns at a time; rows are dealt out
+// in steps of blockDim / 8, so every lane sees about the same number of
+// pairs.
+__device__ inline void eps_far_rows(const MarkerDev& m, const GridScalars& g, int first, int n, int KB,
+                                    int* __restrict__ outside, double* __restrict__ P) {
+    __shared__ int far_sh;
+    const double* X = m.pos + 2 * first;
+    if (threadIdx.x == 0) far_sh = 0;
+    if (P)
+        for (int k = threadIdx.x; k < 2 * n; k += blockDim.x) P[k] = X[k];
+    __syncthreads();
+    if (P) X = P;
+    const double lim = 3.01 * g.Dx;
+    constexpr int CL = 8, U = 4;
+    const int rl = threadIdx.x / CL, cl = threadIdx.x % CL, nrl = blockDim.x / CL;
+    int far = 0;
+    for (int r = rl; r < n; r += nrl) {
+        const double xr = X[2 * r], yr = X[2 * r + 1];
+        for (int c0 = r + KB + 1 + cl; c0 < n; c0 += U * CL) {
+            double xs[U], ys[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = min(c0 + u * CL, n - 1);
+                xs[u] = X[2 * c];
+                ys[u] = X[2 * c + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (c0 + u * CL < n && fabs(xr - xs[u]) < lim && fabs(yr - ys[u]) < lim) far = 1;
+        }
+    }
+    if (far) atomicOr(&far_sh, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *outside = far_sh;
+}
+
+// Launches whose systems have up to this many markers stage the positions
+// for the out-of-band check in LDS.
+constexpr int kFarLdsMarkers = 1024;
+
 // Band entries (the first be workgroups of grid.x) and the out-of-band
-// check (the rest) of each system in one launch: the two are independent
-// and the solve needs both, so the coupling chain waits for one kernel.
+// check (the last workgroup, which also writes the flag) of each system in
+// one launch: the two are independent and the solve needs both, so the
+// coupling chain waits for one kernel.
 __global__ void k_eps_band_prep(MarkerDev m, GridScalars g, const int* __restrict__ sys_first,
                                 const int* __restrict__ sys_n, int KB, double* __restrict__ band,
-                                const long long* __restrict__ band_off, int* __restrict__ outside, int be) {
+                                const long long* __restrict__ band_off, int* __restrict__ outside, int be,
+                                int stage_far) {
+    extern __shared__ double P[];
     coupling_priority();
     const int sys = blockIdx.y;
     const int bx = (int)blockIdx.x;
@@ -209,8 +280,7 @@ __global__ void k_eps_band_prep(MarkerDev m, GridScalars g, const int* __restric
         eps_band_entry(m, g, sys_first[sys], sys_n[sys], KB, band + band_off[sys],
                        (long long)bx * blockDim.x + threadIdx.x);
     else
-        eps_far_pair(m, g, sys_first[sys], sys_n[sys], KB, outside + sys,
-                     (long long)(bx - be) * blockDim.x + threadIdx.x);
+        eps_far_rows(m, g, sys_first[sys], sys_n[sys], KB, outside + sys, stage_far ? P : nullptr);
 }
 
 // Batched assembly: grid.y = entry of the system table.
@@ -483,27 +553,304 @@ __global__ __launch_bounds__(64) void k_eps_band_solve(const int* __restrict__ s
                                                        const int* __restrict__ outside, double* __restrict__ band,
                                                        const long long* __restrict__ band_off,
                                                        double* __restrict__ ywork, double* __restrict__ eps,
-                                                       int lds_rows) {
+                                                       int lds_rows, int skip_upto) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
     __shared__ int bw;
     const int sys = blockIdx.x;
     coupling_priority();
-    if (outside[sys]) return;
+    // skip_upto: systems of up to that many rows are the reduction's
+    if (outside[sys] || sys_n[sys] <= skip_upto) return;
     eps_band_solve_body<KB>(sys_first[sys], sys_n[sys], band + band_off[sys], ywork, eps, lds_rows, sm, xb, &bw);
 }
 
 template <int KB>
 __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __restrict__ cases,
-                                                             const CaseSys* __restrict__ sys, int lds_rows) {
+                                                             const CaseSys* __restrict__ sys, int lds_rows,
+                                                             int skip_upto) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
     __shared__ int bw;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
-    if (c.eps.outside[q.k]) return;
+    if (c.eps.outside[q.k] || c.eps.n[q.k] <= skip_upto) return;
     eps_band_solve_body<KB>(c.eps.first[q.k], c.eps.n[q.k], c.eps.band + c.eps.off[q.k], c.eps.y, c.m.eps, lds_rows,
                             sm, xb, &bw);
+}
+
+// ---------------------------------------------------------------- epsilon by cyclic reduction
+// FP32-shifted cases: the banded epsilon system A eps = 1 of a body solved by
+// block cyclic reduction instead of the pivoted elimination above.  With
+// half-bandwidth bw <= Bk, A is block tridiagonal in Bk x Bk blocks (block
+// row I: rows I*Bk .. I*Bk+Bk-1; padding rows beyond n are identity rows
+// with a zero right-hand side).  At stride s = 1, 2, 4, ... the blocks at
+// odd multiples of s are eliminated from their neighbours' equations, one
+// lane per entry of each surviving block, and back substitution runs the
+// levels in reverse: log2(n / Bk) dependent levels instead of n pivot
+// steps.  A_ij = S_ij ds_j with S the Gram matrix of the markers' delta
+// functions over the lattice (symmetric positive definite for distinct
+// markers), so every Schur complement the reduction forms is S's Schur
+// complement scaled by ds: no pivoting is needed.  The values agree with the
+// pivoted elimination to rounding (the plain method keeps that one, whose
+// operations are the reference's).  Systems of up to kCrMaxRows rows; the
+// kernels leave larger ones to the pivoted solve.
+constexpr int kCrMaxRows = 384;
+
+// LDS (doubles) of the reduction with Bk x Bk blocks over n rows: blocks,
+// staged new blocks of a level, solution.
+__host__ __device__ constexpr size_t cr_lds_doubles(int Bk, int n) {
+    return (size_t)((n + Bk - 1) / Bk) * (3 * Bk * Bk + Bk) + (size_t)(((n + Bk - 1) / Bk + 1) / 2) * (3 * Bk * Bk + Bk) +
+           (size_t)((n + Bk - 1) / Bk) * Bk;
+}
+
+// In-place inverse of a Bk x Bk block (row-major) by Gauss-Jordan
+// elimination without pivoting, in registers.
+template <int Bk>
+__device__ __forceinline__ void cr_inverse(double* __restrict__ D) {
+    double a[Bk][Bk];
+#pragma unroll
+    for (int i = 0; i < Bk; i++)
+#pragma unroll
+        for (int j = 0; j < Bk; j++) a[i][j] = D[Bk * i + j];
+#pragma unroll
+    for (int k = 0; k < Bk; k++) {
+        const double p = 1.0 / a[k][k];
+        a[k][k] = 1.0;
+#pragma unroll
+        for (int j = 0; j < Bk; j++) a[k][j] = a[k][j] * p;
+#pragma unroll
+        for (int i = 0; i < Bk; i++) {
+            if (i == k) continue;
+            const double f = a[i][k];
+            a[i][k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < Bk; j++) a[i][j] = a[i][j] - f * a[k][j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Bk; i++)
+#pragma unroll
+        for (int j = 0; j < Bk; j++) D[Bk * i + j] = a[i][j];
+}
+
+// The reduction of one system by the whole workgroup.  G: the band (KB = 8
+// storage, LDS or global memory), read once into the blocks; blk: LDS for
+// the blocks; work: LDS for the staged blocks and the solution (it may
+// alias G when G is in LDS: G is read only before the first level).
+template <int Bk>
+__device__ __forceinline__ void eps_cr_solve(const double* __restrict__ G, int n, double* __restrict__ eps,
+                                             double* __restrict__ blk, double* __restrict__ work) {
+    constexpr int KB = 8, W = 3 * KB + 1;
+    constexpr int BB = Bk * Bk, BLK = 3 * BB + Bk;  // A (sub), D (diagonal), C (super), f
+    constexpr int oA = 0, oD = BB, oC = 2 * BB, oF = 3 * BB;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nb = (n + Bk - 1) / Bk;
+    // blocks from the band: eight entries per lane at a time, their loads
+    // (unconditional, clamped address) issued before any is used
+    constexpr int U = 8;
+    const int total = nb * BLK;
+    for (int base = 0; base < total; base += U * nt) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * nt + tid;
+            const int I = t / BLK, q = t - I * BLK;
+            const int part = q / BB, i = (q % BB) / Bk, j = q % Bk;
+            const int r = I * Bk + i, c = (I + part - 1) * Bk + j, o = c - r;
+            const bool band_entry = q < oF && r < n && c >= 0 && c < n && o >= -KB && o <= KB;
+            const double a = G[band_entry ? (size_t)r * W + (o + KB) : 0];
+            double val;
+            if (q >= oF) val = I * Bk + (q - oF) < n ? 1.0 : 0.0;
+            else if (r >= n) val = (part == 1 && i == j) ? 1.0 : 0.0;
+            else val = band_entry ? a : 0.0;
+            v[u] = val;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * nt + tid;
+            if (t < total) blk[t] = v[u];
+        }
+    }
+    __syncthreads();
+    double* stage = work;
+    double* x = work + (size_t)((nb + 1) / 2) * BLK;
+    int s = 1;
+    for (; s < nb; s *= 2) {
+        for (int j = s + 2 * s * tid; j < nb; j += 2 * s * nt) cr_inverse<Bk>(blk + (size_t)BLK * j + oD);
+        __syncthreads();
+        const int nr = (nb - 1) / (2 * s) + 1;
+        for (int t = tid; t < BB * nr; t += nt) {
+            const int k = t / BB, r = (t % BB) / Bk, c = t % Bk, i = 2 * s * k;
+            const double* bi = blk + (size_t)BLK * i;
+            double Dn = bi[oD + Bk * r + c], An = 0.0, Cn = 0.0, fn = bi[oF + r];
+            if (i - s >= 0) {
+                const double* bj = blk + (size_t)BLK * (i - s);
+                double al[Bk];
+#pragma unroll
+                for (int q = 0; q < Bk; q++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int m = 0; m < Bk; m++) a += bi[oA + Bk * r + m] * bj[oD + Bk * m + q];
+                    al[q] = a;
+                }
+                double dc = 0.0, da = 0.0, df = 0.0;
+#pragma unroll
+                for (int q = 0; q < Bk; q++) {
+                    dc += al[q] * bj[oC + Bk * q + c];
+                    da += al[q] * bj[oA + Bk * q + c];
+                    df += al[q] * bj[oF + q];
+                }
+                Dn = Dn - dc;
+                An = -da;
+                fn = fn - df;
+            }
+            if (i + s < nb) {
+                const double* bj = blk + (size_t)BLK * (i + s);
+                double ga[Bk];
+#pragma unroll
+                for (int q = 0; q < Bk; q++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int m = 0; m < Bk; m++) a += bi[oC + Bk * r + m] * bj[oD + Bk * m + q];
+                    ga[q] = a;
+                }
+                double da = 0.0, dc = 0.0, df = 0.0;
+#pragma unroll
+                for (int q = 0; q < Bk; q++) {
+                    da += ga[q] * bj[oA + Bk * q + c];
+                    dc += ga[q] * bj[oC + Bk * q + c];
+                    df += ga[q] * bj[oF + q];
+                }
+                Dn = Dn - da;
+                Cn = -dc;
+                fn = fn - df;
+            }
+            double* o = stage + (size_t)BLK * k;
+            o[oA + Bk * r + c] = An;
+            o[oD + Bk * r + c] = Dn;
+            o[oC + Bk * r + c] = Cn;
+            if (c == 0) o[oF + r] = fn;
+        }
+        __syncthreads();
+        for (int t = tid; t < BLK * nr; t += nt) blk[(size_t)BLK * 2 * s * (t / BLK) + t % BLK] = stage[t];
+        __syncthreads();
+    }
+    // block 0 alone: x_0 = inv(D_0) f_0
+    if (tid == 0) cr_inverse<Bk>(blk + oD);
+    __syncthreads();
+    if (tid < Bk) {
+        double a = 0.0;
+#pragma unroll
+        for (int m = 0; m < Bk; m++) a += blk[oD + Bk * tid + m] * blk[oF + m];
+        x[tid] = a;
+    }
+    __syncthreads();
+    // back substitution: x_j = inv(D_j) (f_j - A_j x_{j-s} - C_j x_{j+s})
+    for (s /= 2; s >= 1; s /= 2) {
+        const int ne = (nb + s - 1) / (2 * s);
+        for (int t = tid; t < Bk * ne; t += nt) {
+            const int j = s + 2 * s * (t / Bk), r = t % Bk;
+            const double* bj = blk + (size_t)BLK * j;
+            const double* xl = x + (size_t)Bk * (j - s);
+            const bool right = j + s < nb;
+            const double* xr = x + (size_t)Bk * (right ? j + s : j - s);
+            double v[Bk];
+#pragma unroll
+            for (int a = 0; a < Bk; a++) {
+                double ax = 0.0, cx = 0.0;
+#pragma unroll
+                for (int m = 0; m < Bk; m++) {
+                    ax += bj[oA + Bk * a + m] * xl[m];
+                    cx += bj[oC + Bk * a + m] * xr[m];
+                }
+                v[a] = bj[oF + a] - ax - (right ? cx : 0.0);
+            }
+            double xv = 0.0;
+#pragma unroll
+            for (int m = 0; m < Bk; m++) xv += bj[oD + Bk * r + m] * v[m];
+            x[(size_t)Bk * j + r] = xv;
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < n; k += nt) eps[k] = x[k];
+}
+
+// Block size for a measured half-bandwidth.
+__device__ __forceinline__ void eps_cr_dispatch(int bw, const double* G, int n, double* eps, double* blk,
+                                                double* work) {
+    if (bw <= 2) eps_cr_solve<2>(G, n, eps, blk, work);
+    else if (bw <= 4) eps_cr_solve<4>(G, n, eps, blk, work);
+    else eps_cr_solve<8>(G, n, eps, blk, work);
+}
+
+// Half-bandwidth of a system's band in global memory (largest |column - row|
+// over its non-zero entries), reduced over the workgroup.
+__device__ __forceinline__ int band_width_wg(const double* __restrict__ G, int n, int* bw_sh) {
+    constexpr int KB = 8, W = 3 * KB + 1;
+    if (threadIdx.x == 0) *bw_sh = 0;
+    __syncthreads();
+    // sixteen loads in flight per lane (clamped addresses)
+    constexpr int U = 16;
+    const int total = n * W, nt = blockDim.x;
+    int bw = 0;
+    for (int base = 0; base < total; base += U * nt) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = G[min(base + u * nt + (int)threadIdx.x, total - 1)];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * nt + (int)threadIdx.x;
+            const int q = t % W;
+            if (t < total && q <= 2 * KB && v[u] != 0.0) bw = max(bw, q < KB ? KB - q : q - KB);
+        }
+    }
+    atomicMax(bw_sh, bw);
+    __syncthreads();
+    return *bw_sh;
+}
+
+// LDS of the reduction kernels for systems of up to max_n rows (blocks of
+// the widest size, 8).
+static size_t eps_cr_lds(int max_n) {
+    const int n = max_n < kCrMaxRows ? max_n : kCrMaxRows;
+    return sizeof(double) * cr_lds_doubles(8, n);
+}
+
+// Dynamic LDS beyond 64 KiB per workgroup (gfx950 has 160 KiB) for a kernel,
+// allowed once per process.
+static void allow_lds(const void* kernel, size_t bytes) {
+    if (bytes > 64 * 1024 &&
+        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+        throw std::runtime_error("hipFuncSetAttribute: dynamic LDS of the epsilon solve");
+}
+
+__global__ __launch_bounds__(256) void k_eps_band_cr(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
+                                                     const int* __restrict__ outside, const double* __restrict__ band,
+                                                     const long long* __restrict__ band_off,
+                                                     double* __restrict__ eps) {
+    extern __shared__ double sm[];
+    __shared__ int bw_sh;
+    const int sys = blockIdx.x, n = sys_n[sys];
+    coupling_priority();
+    if (outside[sys] || n > kCrMaxRows) return;
+    const double* G = band + band_off[sys];
+    const int bw = band_width_wg(G, n, &bw_sh);
+    const int nb8 = (n + 7) / 8;
+    eps_cr_dispatch(bw, G, n, eps + sys_first[sys], sm, sm + (size_t)nb8 * (3 * 64 + 8));
+}
+
+__global__ __launch_bounds__(256) void k_eps_band_cr_batch(const CaseDev* __restrict__ cases,
+                                                           const CaseSys* __restrict__ sys) {
+    extern __shared__ double sm[];
+    __shared__ int bw_sh;
+    const CaseSys q = sys[blockIdx.x];
+    const CaseDev& c = cases[q.c];
+    const int n = c.eps.n[q.k];
+    if (c.eps.outside[q.k] || n > kCrMaxRows) return;
+    const double* G = c.eps.band + c.eps.off[q.k];
+    const int bw = band_width_wg(G, n, &bw_sh);
+    const int nb8 = (n + 7) / 8;
+    eps_cr_dispatch(bw, G, n, c.m.eps + c.eps.first[q.k], sm, sm + (size_t)nb8 * (3 * 64 + 8));
 }
 
 // ---------------------------------------------------------------- fused recompute
@@ -528,7 +875,7 @@ template <int KB>
 __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridScalars& g, int first, int n,
                                                int* __restrict__ outside, double* __restrict__ sm,
                                                double* __restrict__ xb, int* __restrict__ flag,
-                                               int* __restrict__ bw_sh) {
+                                               int* __restrict__ bw_sh, bool cr) {
     constexpr int W = 3 * KB + 1;
     double* P = sm;                // 2n positions
     double* dsl = sm + 2 * n;      // n spacings
@@ -593,25 +940,34 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     __syncthreads();
     const int far = *flag;
     if (tid == 0) *outside = far;
-    if (far || tid >= 64) return;
+    if (far) return;
+    if (cr) {
+        // the reduction by the whole workgroup, its blocks after the band
+        double* blk = yl + n;
+        eps_cr_dispatch(*bw_sh, Bl, n, m.eps + first, blk, blk + (size_t)((n + 7) / 8) * (3 * 64 + 8));
+        return;
+    }
+    if (tid >= 64) return;
     eps_band_solve_bw<KB, true, true>(*bw_sh, first, n, Bl, yl, m.eps, xb);
 }
 
 // Lanes per body workgroup (at least one wavefront for the solve).  Beside
 // the early bulk kernel, 256, 512 and 1024 measured within 3% of each other
 // on MI355X and 128 8% slower (environment LIFE_SERVE_FUSED_THREADS = 64 |
-// 128 | 256 | 512 | 1024 for sweeps, 0 for the separate batched kernels).
+// 128 | 256 | 512 for sweeps, 0 for the separate batched kernels).  At most
+// 512, so that a lane has the registers for the cyclic reduction's 8 x 8
+// block inverse without spilling.
 constexpr int kFusedThreads = 512;
 
 template <int KB>
-__global__ __launch_bounds__(1024) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
-                                                              const CaseSys* __restrict__ sys) {
+__global__ __launch_bounds__(512) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
+                                                              const CaseSys* __restrict__ sys, int cr) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
     __shared__ int flag, bw;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
-    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag, &bw);
+    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag, &bw, cr != 0);
 }
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
@@ -840,19 +1196,28 @@ static int eps_lds_rows(int KB, int max_n) {
 
 void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
                      const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
-                     double* ywork, hipStream_t s) {
+                     double* ywork, bool cr, hipStream_t s) {
     if (nsys <= 0 || KB != 8) return;
     int max_n = 0;
     for (int k = 0; k < nsys; k++) max_n = max_n > host_n[k] ? max_n : host_n[k];
     const int W = 3 * KB + 1;
-    const long long ne = (long long)max_n * W, np = (long long)max_n * max_n;
-    const int be = (int)((ne + 255) / 256), bf = (int)((np + 255) / 256);
-    k_eps_band_prep<<<dim3((unsigned)(be + bf), nsys), 256, 0, s>>>(m, g, sys_first, sys_n, KB, band, band_off, outside,
-                                                                     be);
+    const long long ne = (long long)max_n * W;
+    const int be = (int)((ne + 255) / 256);
+    const bool stage_far = max_n <= kFarLdsMarkers;
+    const size_t far_lds = stage_far ? sizeof(double) * 2 * (size_t)max_n : 0;
+    k_eps_band_prep<<<dim3((unsigned)(be + 1), nsys), 256, far_lds, s>>>(m, g, sys_first, sys_n, KB, band, band_off,
+                                                                          outside, be, stage_far ? 1 : 0);
+    if (cr) {
+        const size_t lds = eps_cr_lds(max_n);
+        static const bool once = (allow_lds((const void*)k_eps_band_cr, eps_cr_lds(kCrMaxRows)), true);
+        (void)once;
+        k_eps_band_cr<<<nsys, 256, lds, s>>>(sys_first, sys_n, outside, band, band_off, m.eps);
+        if (max_n <= kCrMaxRows) return;
+    }
     // band, then right-hand side, in LDS for systems of up to lds_rows rows
     const int rows = eps_lds_rows(KB, max_n);
-    k_eps_band_solve<8><<<nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(sys_first, sys_n, outside, band,
-                                                                                   band_off, ywork, m.eps, rows);
+    k_eps_band_solve<8><<<nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(
+        sys_first, sys_n, outside, band, band_off, ywork, m.eps, rows, cr ? kCrMaxRows : 0);
 }
 
 void launch_find_support(const MarkerDev& m, const GridScalars& g, int flexible_only, hipStream_t s) {
@@ -916,7 +1281,7 @@ void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int m
 }
 
 void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
-                           hipStream_t s) {
+                           bool cr, hipStream_t s) {
     constexpr int KB = 8, W = 3 * KB + 1;
     if (nsys <= 0 || max_n <= 0) return;
     const unsigned nb = (unsigned)((nsys + 255) / 256);
@@ -924,8 +1289,16 @@ void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, i
     const long long ne = (long long)max_n * W, np = (long long)max_n * max_n;
     k_eps_band_entries_batch<<<dim3((unsigned)((ne + 255) / 256), (unsigned)nsys), 256, 0, s>>>(cases, sys, KB);
     k_eps_far_check_batch<<<dim3((unsigned)((np + 255) / 256), (unsigned)nsys), 256, 0, s>>>(cases, sys, KB);
-    const int rows = eps_lds_rows(KB, max_n);
-    k_eps_band_solve_batch<KB><<<(unsigned)nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(cases, sys, rows);
+    if (cr) {
+        static const bool once = (allow_lds((const void*)k_eps_band_cr_batch, eps_cr_lds(kCrMaxRows)), true);
+        (void)once;
+        k_eps_band_cr_batch<<<(unsigned)nsys, 256, eps_cr_lds(max_n), s>>>(cases, sys);
+    }
+    if (!cr || max_n > kCrMaxRows) {
+        const int rows = eps_lds_rows(KB, max_n);
+        k_eps_band_solve_batch<KB><<<(unsigned)nsys, 64, sizeof(double) * (size_t)rows * (W + 1), s>>>(
+            cases, sys, rows, cr ? kCrMaxRows : 0);
+    }
     k_eps_outside_gather_batch<<<nb, 256, 0, s>>>(cases, sys, nsys, outside_out);
 }
 
@@ -965,19 +1338,25 @@ void launch_markers_out_batch(const CaseDev* cases, const int* list, int n, int 
 // Fused recompute (one workgroup per flexible body); false when a body has
 // more markers than the LDS layout takes (the caller uses the separate
 // kernels then).
-static size_t fused_recompute_lds(int max_n) { return sizeof(double) * (size_t)max_n * (3 * 8 + 5); }
+static size_t fused_recompute_lds(int max_n, bool cr) {
+    return sizeof(double) * ((size_t)max_n * (3 * 8 + 5) + (cr ? cr_lds_doubles(8, max_n) : 0));
+}
 
 bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
-                                  hipStream_t s) {
+                                  bool cr, hipStream_t s) {
     if (max_n > kFusedMaxMarkers) return false;
     if (nsys <= 0) return true;
     static const int nt = [] {
         const char* e = std::getenv("LIFE_SERVE_FUSED_THREADS");
         const int v = e ? std::atoi(e) : kFusedThreads;
-        return v == 0 || v == 64 || v == 128 || v == 256 || v == 1024 ? v : kFusedThreads;
+        return v == 0 || v == 64 || v == 128 || v == 256 ? v : kFusedThreads;
     }();
     if (nt == 0) return false;  // the separate batched kernels
-    k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n), s>>>(cases, sys);
+    static const bool once =
+        (allow_lds((const void*)k_recompute_body_batch<8>, fused_recompute_lds(kFusedMaxMarkers, true)), true);
+    (void)once;
+    k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n, cr), s>>>(cases, sys,
+                                                                                                  cr ? 1 : 0);
     k_eps_outside_gather_batch<<<(unsigned)((nsys + 255) / 256), 256, 0, s>>>(cases, sys, nsys, outside_out);
     return true;
 }

@@ -91,11 +91,13 @@ void launch_support_ds(const MarkerDev& m, const GridScalars& g, int nbodies, in
 void launch_compute_ds(const MarkerDev& m, const GridScalars& g, int nbodies, int flexible_only, hipStream_t s);
 void launch_epsilon_matrix(const MarkerDev& m, const GridScalars& g, int first, int n, double* A, hipStream_t s);
 // Banded epsilon systems (one per entry of sys_first/sys_n), assembled and
-// solved on the device into m.eps; outside[k] = 1 marks a system whose
-// matrix is not confined to the band (left for the dense host path).
+// solved on the device into m.eps; outside[k] is written: 1 for a system whose
+// matrix is not confined to the band (left for the dense host path).  cr:
+// systems of up to kCrMaxRows rows by block cyclic reduction (FP32-shifted
+// cases), else the pivoted elimination.
 void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_first, const int* sys_n,
                      const int* host_n, int nsys, int KB, double* band, const long long* band_off, int* outside,
-                     double* ywork, hipStream_t s);
+                     double* ywork, bool cr, hipStream_t s);
 template <typename T>
 void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s);
 void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,
@@ -174,16 +176,17 @@ void launch_find_support_batch(const CaseDev* cases, const int* list, int n, int
                                hipStream_t s);
 void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int max_bodies, int flexible_only,
                              hipStream_t s);
-// outside_out[q] = 1 for a listed system that is not banded (no solve done)
+// outside_out[q] = 1 for a listed system that is not banded (no solve done);
+// cr as for launch_eps_band
 void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
-                           hipStream_t s);
+                           bool cr, hipStream_t s);
 // Supports and spacing of the listed systems' markers plus their banded
 // epsilon systems, one workgroup per system (the work of
 // launch_find_support_batch(flexible), launch_compute_ds_batch and
 // launch_eps_band_batch in one launch).  Returns false, launching nothing,
 // if a system has more markers than the fused kernel takes.
 bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
-                                  hipStream_t s);
+                                  bool cr, hipStream_t s);
 template <typename T>
 void launch_interpolate_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);
 void launch_spread_batch(const CaseDev* cases, const int* list, int n, int max_markers, hipStream_t s);

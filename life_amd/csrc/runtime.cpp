@@ -511,14 +511,17 @@ void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const s
     LIFE_HIP(hipStreamSynchronize(g->stream));
 }
 
-void ibm_epsilon_band_launch(life_grid* g) {
+void ibm_epsilon_band_launch(life_grid* g, bool copy_flags) {
     if (g->eps_nsys <= 0) return;
-    launch_fill_i32(g->eps_outside, 0, g->eps_nsys, g->stream);
+    // (the band kernels write every system's out-of-band flag)
+    // FP32-shifted cases: cyclic reduction; the plain method: the pivoted
+    // elimination in the reference's order
     launch_eps_band(g->md, g->g, g->eps_first, g->eps_n, g->eps_n_h.data(), g->eps_nsys, g->eps_KB, g->eps_band, g->eps_off,
-                    g->eps_outside, g->eps_y, g->stream);
+                    g->eps_outside, g->eps_y, g->prec == LIFE_FP32_SHIFTED, g->stream);
     LIFE_HIP(hipGetLastError());
-    LIFE_HIP(hipMemcpyAsync(g->eps_outside_h, g->eps_outside, sizeof(int) * g->eps_nsys, hipMemcpyDeviceToHost,
-                            g->stream));
+    if (copy_flags)
+        LIFE_HIP(hipMemcpyAsync(g->eps_outside_h, g->eps_outside, sizeof(int) * g->eps_nsys, hipMemcpyDeviceToHost,
+                                g->stream));
 }
 
 std::vector<int> ibm_epsilon_band_dense(const life_grid* g) {
@@ -649,6 +652,14 @@ void handle_release() {
     if (--g_handles == 0) release_pools(false);
 }
 
+void grid_reserve_coupling_cus(life_grid* g, int cus) {
+    if (std::getenv("LIFE_COUPLING_CUS")) return;  // the environment's choice stands
+    hipStream_t s = masked_stream(g->device, cus);
+    if (s == g->stream2) return;
+    LIFE_HIP(hipStreamSynchronize(g->stream2));
+    g->stream2 = s;
+}
+
 }  // namespace life
 
 using namespace life;
@@ -707,6 +718,7 @@ void init_coupling_priority(int device) {
 // The second stream of a grid: the early bulk stream-collide.
 hipStream_t bulk_stream(int device) { return masked_stream(device, coupling_cus()); }
 }  // namespace
+
 
 
 int life_grid_create(const life_params* p, int precision, int device, life_grid** out) {

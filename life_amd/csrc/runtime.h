@@ -150,6 +150,9 @@ void lbm_join(life_grid* g);
 // One per (device, reserve) for the whole process, never destroyed: a
 // CU-masked stream holds a hardware queue of its own.
 hipStream_t masked_stream(int device, int reserve_cus);
+// The grid's bulk stream-collide keeps cus CUs free for the coupling kernels
+// (unless the environment sets LIFE_COUPLING_CUS); before any step.
+void grid_reserve_coupling_cus(life_grid* g, int cus);
 // Count of live grids and servers; the last release frees every pooled HIP
 // resource (idle streams, kept device/pinned blocks, CU-masked streams).
 void handle_acquire();
@@ -175,9 +178,10 @@ void ibm_find_support(life_grid* g, int flexible_only);
 void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host);
 // Declare the banded epsilon systems [first[k], first[k]+n[k]) once.
 void ibm_epsilon_band_setup(life_grid* g, const std::vector<int>& first, const std::vector<int>& n, int KB);
-// Assemble and solve them on the device (epsilon lands in md.eps) and queue
-// the copy of their out-of-band flags to the host; does not wait.
-void ibm_epsilon_band_launch(life_grid* g);
+// Assemble and solve them on the device (epsilon lands in md.eps) and, with
+// copy_flags, queue the copy of their out-of-band flags to eps_outside_h
+// (else the caller brings them over); does not wait.
+void ibm_epsilon_band_launch(life_grid* g, bool copy_flags = true);
 // After the grid stream has been synchronised past ibm_epsilon_band_launch:
 // the indices of systems that were not banded (not solved; their epsilon
 // still needs a solve).

@@ -685,14 +685,17 @@ void coupling_phase(life_server* s) {
             }
             // one workgroup per flexible body for supports, spacing and the
             // banded epsilon system when every case can take that path
+            // FP32-shifted cases solve their banded systems by cyclic
+            // reduction, as life_case does
+            const bool cr = s->cfg.precision == LIFE_FP32_SHIFTED;
             const bool fused = !any_uni && !sysl.empty() &&
                                launch_recompute_fused_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n,
-                                                            s->outside.d, st);
+                                                            s->outside.d, cr, st);
             if (!fused) {
                 launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
                 launch_compute_ds_batch(s->cases.d, s->dyn.d, na, s->max_bodies, 1, st);
                 if (!sysl.empty())
-                    launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, st);
+                    launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, cr, st);
             }
             if (!sysl.empty()) s->outside.download(sysl.size(), st);
         }
