@@ -592,9 +592,12 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
 // markers), so every Schur complement the reduction forms is S's Schur
 // complement scaled by ds: no pivoting is needed.  The values agree with the
 // pivoted elimination to rounding (the plain method keeps that one, whose
-// operations are the reference's).  Systems of up to kCrMaxRows rows; the
-// kernels leave larger ones to the pivoted solve.
+// operations are the reference's).  Systems of up to kCrMaxRows rows and
+// half-bandwidth up to kCrMaxBw (blocks of 2 x 2 or 4 x 4); a wider band is
+// solved by the pivoted elimination on wavefront 0 of the same kernel, and
+// larger systems by the pivoted kernel.
 constexpr int kCrMaxRows = 384;
+constexpr int kCrMaxBw = 4;
 
 // LDS (doubles) of the reduction with Bk x Bk blocks over n rows: blocks,
 // staged new blocks of a level, solution.
@@ -775,12 +778,11 @@ __device__ __forceinline__ void eps_cr_solve(const double* __restrict__ G, int n
     for (int k = tid; k < n; k += nt) eps[k] = x[k];
 }
 
-// Block size for a measured half-bandwidth.
+// Block size for a measured half-bandwidth (at most kCrMaxBw).
 __device__ __forceinline__ void eps_cr_dispatch(int bw, const double* G, int n, double* eps, double* blk,
                                                 double* work) {
     if (bw <= 2) eps_cr_solve<2>(G, n, eps, blk, work);
-    else if (bw <= 4) eps_cr_solve<4>(G, n, eps, blk, work);
-    else eps_cr_solve<8>(G, n, eps, blk, work);
+    else eps_cr_solve<4>(G, n, eps, blk, work);
 }
 
 // Half-bandwidth of a system's band in global memory (largest |column - row|
@@ -810,10 +812,10 @@ __device__ __forceinline__ int band_width_wg(const double* __restrict__ G, int n
 }
 
 // LDS of the reduction kernels for systems of up to max_n rows (blocks of
-// the widest size, 8).
+// the widest size, 4).
 static size_t eps_cr_lds(int max_n) {
     const int n = max_n < kCrMaxRows ? max_n : kCrMaxRows;
-    return sizeof(double) * cr_lds_doubles(8, n);
+    return sizeof(double) * cr_lds_doubles(kCrMaxBw, n);
 }
 
 // Dynamic LDS beyond 64 KiB per workgroup (gfx950 has 160 KiB) for a kernel,
@@ -824,33 +826,45 @@ static void allow_lds(const void* kernel, size_t bytes) {
         throw std::runtime_error("hipFuncSetAttribute: dynamic LDS of the epsilon solve");
 }
 
+// One system per workgroup: the reduction, or for a band wider than
+// kCrMaxBw the pivoted elimination on the band in device memory (ywork: its
+// right-hand side).  That elimination's lanes are those of wavefront 0; the
+// other wavefronts run along without storing anything and meet it at the
+// workgroup barrier before the back substitution.
+__device__ __forceinline__ void eps_cr_or_pivot(double* __restrict__ G, int first, int n, double* __restrict__ ywork,
+                                                double* __restrict__ eps, double* sm, double* xb, int* bw_sh) {
+    const int bw = band_width_wg(G, n, bw_sh);
+    if (bw > kCrMaxBw) {
+        eps_band_solve_bw<8, false>(bw, first, n, G, ywork + first, eps, xb);
+        return;
+    }
+    const int nb = (n + kCrMaxBw - 1) / kCrMaxBw;
+    eps_cr_dispatch(bw, G, n, eps + first, sm, sm + (size_t)nb * (3 * kCrMaxBw * kCrMaxBw + kCrMaxBw));
+}
+
 __global__ __launch_bounds__(256) void k_eps_band_cr(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
-                                                     const int* __restrict__ outside, const double* __restrict__ band,
+                                                     const int* __restrict__ outside, double* __restrict__ band,
                                                      const long long* __restrict__ band_off,
-                                                     double* __restrict__ eps) {
+                                                     double* __restrict__ ywork, double* __restrict__ eps) {
     extern __shared__ double sm[];
+    __shared__ double xb[2 * (8 + 1)];
     __shared__ int bw_sh;
     const int sys = blockIdx.x, n = sys_n[sys];
     coupling_priority();
     if (outside[sys] || n > kCrMaxRows) return;
-    const double* G = band + band_off[sys];
-    const int bw = band_width_wg(G, n, &bw_sh);
-    const int nb8 = (n + 7) / 8;
-    eps_cr_dispatch(bw, G, n, eps + sys_first[sys], sm, sm + (size_t)nb8 * (3 * 64 + 8));
+    eps_cr_or_pivot(band + band_off[sys], sys_first[sys], n, ywork, eps, sm, xb, &bw_sh);
 }
 
 __global__ __launch_bounds__(256) void k_eps_band_cr_batch(const CaseDev* __restrict__ cases,
                                                            const CaseSys* __restrict__ sys) {
     extern __shared__ double sm[];
+    __shared__ double xb[2 * (8 + 1)];
     __shared__ int bw_sh;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
     const int n = c.eps.n[q.k];
     if (c.eps.outside[q.k] || n > kCrMaxRows) return;
-    const double* G = c.eps.band + c.eps.off[q.k];
-    const int bw = band_width_wg(G, n, &bw_sh);
-    const int nb8 = (n + 7) / 8;
-    eps_cr_dispatch(bw, G, n, c.m.eps + c.eps.first[q.k], sm, sm + (size_t)nb8 * (3 * 64 + 8));
+    eps_cr_or_pivot(c.eps.band + c.eps.off[q.k], c.eps.first[q.k], n, c.eps.y, c.m.eps, sm, xb, &bw_sh);
 }
 
 // ---------------------------------------------------------------- fused recompute
@@ -941,10 +955,12 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     const int far = *flag;
     if (tid == 0) *outside = far;
     if (far) return;
-    if (cr) {
+    if (cr && *bw_sh <= kCrMaxBw) {
         // the reduction by the whole workgroup, its blocks after the band
         double* blk = yl + n;
-        eps_cr_dispatch(*bw_sh, Bl, n, m.eps + first, blk, blk + (size_t)((n + 7) / 8) * (3 * 64 + 8));
+        const int nb = (n + kCrMaxBw - 1) / kCrMaxBw;
+        eps_cr_dispatch(*bw_sh, Bl, n, m.eps + first, blk,
+                        blk + (size_t)nb * (3 * kCrMaxBw * kCrMaxBw + kCrMaxBw));
         return;
     }
     if (tid >= 64) return;
@@ -954,9 +970,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
 // Lanes per body workgroup (at least one wavefront for the solve).  Beside
 // the early bulk kernel, 256, 512 and 1024 measured within 3% of each other
 // on MI355X and 128 8% slower (environment LIFE_SERVE_FUSED_THREADS = 64 |
-// 128 | 256 | 512 for sweeps, 0 for the separate batched kernels).  At most
-// 512, so that a lane has the registers for the cyclic reduction's 8 x 8
-// block inverse without spilling.
+// 128 | 256 | 512 for sweeps, 0 for the separate batched kernels).
 constexpr int kFusedThreads = 512;
 
 template <int KB>
@@ -1211,7 +1225,7 @@ void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_fi
         const size_t lds = eps_cr_lds(max_n);
         static const bool once = (allow_lds((const void*)k_eps_band_cr, eps_cr_lds(kCrMaxRows)), true);
         (void)once;
-        k_eps_band_cr<<<nsys, 256, lds, s>>>(sys_first, sys_n, outside, band, band_off, m.eps);
+        k_eps_band_cr<<<nsys, 256, lds, s>>>(sys_first, sys_n, outside, band, band_off, ywork, m.eps);
         if (max_n <= kCrMaxRows) return;
     }
     // band, then right-hand side, in LDS for systems of up to lds_rows rows
@@ -1339,7 +1353,7 @@ void launch_markers_out_batch(const CaseDev* cases, const int* list, int n, int 
 // more markers than the LDS layout takes (the caller uses the separate
 // kernels then).
 static size_t fused_recompute_lds(int max_n, bool cr) {
-    return sizeof(double) * ((size_t)max_n * (3 * 8 + 5) + (cr ? cr_lds_doubles(8, max_n) : 0));
+    return sizeof(double) * ((size_t)max_n * (3 * 8 + 5) + (cr ? cr_lds_doubles(kCrMaxBw, max_n) : 0));
 }
 
 bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
