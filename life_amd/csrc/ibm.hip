@@ -595,9 +595,20 @@ __global__ __launch_bounds__(64) void k_eps_band_solve_batch(const CaseDev* __re
 // operations are the reference's).  Systems of up to kCrMaxRows rows and
 // half-bandwidth up to kCrMaxBw (blocks of 2 x 2 or 4 x 4); a wider band is
 // solved by the pivoted elimination on wavefront 0 of the same kernel, and
-// larger systems by the pivoted kernel.
+// larger systems by the pivoted kernel.  The markers of a filament lie one
+// lattice spacing apart, which bounds the half-bandwidth at 4; a lower limit
+// (cr_max_bw) sends narrower bands to the fallback as well.
 constexpr int kCrMaxRows = 384;
 constexpr int kCrMaxBw = 4;
+
+// Widest half-bandwidth the reduction takes: environment LIFE_EPS_CR_MAX_BW
+// (0 .. 4, default 4; 0: every system by the pivoted elimination), read at
+// each launch.
+static int cr_max_bw() {
+    const char* e = std::getenv("LIFE_EPS_CR_MAX_BW");
+    const int v = e && *e ? std::atoi(e) : kCrMaxBw;
+    return v < 0 ? 0 : (v > kCrMaxBw ? kCrMaxBw : v);
+}
 
 // LDS (doubles) of the reduction with Bk x Bk blocks over n rows: blocks,
 // staged new blocks of a level, solution.
@@ -832,9 +843,10 @@ static void allow_lds(const void* kernel, size_t bytes) {
 // other wavefronts run along without storing anything and meet it at the
 // workgroup barrier before the back substitution.
 __device__ __forceinline__ void eps_cr_or_pivot(double* __restrict__ G, int first, int n, double* __restrict__ ywork,
-                                                double* __restrict__ eps, double* sm, double* xb, int* bw_sh) {
+                                                double* __restrict__ eps, double* sm, double* xb, int* bw_sh,
+                                                int maxbw) {
     const int bw = band_width_wg(G, n, bw_sh);
-    if (bw > kCrMaxBw) {
+    if (bw > maxbw) {
         eps_band_solve_bw<8, false>(bw, first, n, G, ywork + first, eps, xb);
         return;
     }
@@ -845,18 +857,18 @@ __device__ __forceinline__ void eps_cr_or_pivot(double* __restrict__ G, int firs
 __global__ __launch_bounds__(256) void k_eps_band_cr(const int* __restrict__ sys_first, const int* __restrict__ sys_n,
                                                      const int* __restrict__ outside, double* __restrict__ band,
                                                      const long long* __restrict__ band_off,
-                                                     double* __restrict__ ywork, double* __restrict__ eps) {
+                                                     double* __restrict__ ywork, double* __restrict__ eps, int maxbw) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (8 + 1)];
     __shared__ int bw_sh;
     const int sys = blockIdx.x, n = sys_n[sys];
     coupling_priority();
     if (outside[sys] || n > kCrMaxRows) return;
-    eps_cr_or_pivot(band + band_off[sys], sys_first[sys], n, ywork, eps, sm, xb, &bw_sh);
+    eps_cr_or_pivot(band + band_off[sys], sys_first[sys], n, ywork, eps, sm, xb, &bw_sh, maxbw);
 }
 
 __global__ __launch_bounds__(256) void k_eps_band_cr_batch(const CaseDev* __restrict__ cases,
-                                                           const CaseSys* __restrict__ sys) {
+                                                           const CaseSys* __restrict__ sys, int maxbw) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (8 + 1)];
     __shared__ int bw_sh;
@@ -864,7 +876,7 @@ __global__ __launch_bounds__(256) void k_eps_band_cr_batch(const CaseDev* __rest
     const CaseDev& c = cases[q.c];
     const int n = c.eps.n[q.k];
     if (c.eps.outside[q.k] || n > kCrMaxRows) return;
-    eps_cr_or_pivot(c.eps.band + c.eps.off[q.k], c.eps.first[q.k], n, c.eps.y, c.m.eps, sm, xb, &bw_sh);
+    eps_cr_or_pivot(c.eps.band + c.eps.off[q.k], c.eps.first[q.k], n, c.eps.y, c.m.eps, sm, xb, &bw_sh, maxbw);
 }
 
 // ---------------------------------------------------------------- fused recompute
@@ -889,7 +901,7 @@ template <int KB>
 __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridScalars& g, int first, int n,
                                                int* __restrict__ outside, double* __restrict__ sm,
                                                double* __restrict__ xb, int* __restrict__ flag,
-                                               int* __restrict__ bw_sh, bool cr) {
+                                               int* __restrict__ bw_sh, int cr_bw) {
     constexpr int W = 3 * KB + 1;
     double* P = sm;                // 2n positions
     double* dsl = sm + 2 * n;      // n spacings
@@ -955,7 +967,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
     const int far = *flag;
     if (tid == 0) *outside = far;
     if (far) return;
-    if (cr && *bw_sh <= kCrMaxBw) {
+    if (*bw_sh <= cr_bw) {
         // the reduction by the whole workgroup, its blocks after the band
         double* blk = yl + n;
         const int nb = (n + kCrMaxBw - 1) / kCrMaxBw;
@@ -975,13 +987,13 @@ constexpr int kFusedThreads = 512;
 
 template <int KB>
 __global__ __launch_bounds__(512) void k_recompute_body_batch(const CaseDev* __restrict__ cases,
-                                                              const CaseSys* __restrict__ sys, int cr) {
+                                                              const CaseSys* __restrict__ sys, int cr_bw) {
     extern __shared__ double sm[];
     __shared__ double xb[2 * (KB + 1)];
     __shared__ int flag, bw;
     const CaseSys q = sys[blockIdx.x];
     const CaseDev& c = cases[q.c];
-    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag, &bw, cr != 0);
+    recompute_body<KB>(c.m, c.g, c.eps.first[q.k], c.eps.n[q.k], c.eps.outside + q.k, sm, xb, &flag, &bw, cr_bw);
 }
 
 // ObjectsClass::ibmKernelInterp body (IBMNode.cpp:26-58): rho and rho*u at
@@ -1225,7 +1237,7 @@ void launch_eps_band(const MarkerDev& m, const GridScalars& g, const int* sys_fi
         const size_t lds = eps_cr_lds(max_n);
         static const bool once = (allow_lds((const void*)k_eps_band_cr, eps_cr_lds(kCrMaxRows)), true);
         (void)once;
-        k_eps_band_cr<<<nsys, 256, lds, s>>>(sys_first, sys_n, outside, band, band_off, ywork, m.eps);
+        k_eps_band_cr<<<nsys, 256, lds, s>>>(sys_first, sys_n, outside, band, band_off, ywork, m.eps, cr_max_bw());
         if (max_n <= kCrMaxRows) return;
     }
     // band, then right-hand side, in LDS for systems of up to lds_rows rows
@@ -1306,7 +1318,7 @@ void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, i
     if (cr) {
         static const bool once = (allow_lds((const void*)k_eps_band_cr_batch, eps_cr_lds(kCrMaxRows)), true);
         (void)once;
-        k_eps_band_cr_batch<<<(unsigned)nsys, 256, eps_cr_lds(max_n), s>>>(cases, sys);
+        k_eps_band_cr_batch<<<(unsigned)nsys, 256, eps_cr_lds(max_n), s>>>(cases, sys, cr_max_bw());
     }
     if (!cr || max_n > kCrMaxRows) {
         const int rows = eps_lds_rows(KB, max_n);
@@ -1369,8 +1381,9 @@ bool launch_recompute_fused_batch(const CaseDev* cases, const CaseSys* sys, int 
     static const bool once =
         (allow_lds((const void*)k_recompute_body_batch<8>, fused_recompute_lds(kFusedMaxMarkers, true)), true);
     (void)once;
-    k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n, cr), s>>>(cases, sys,
-                                                                                                  cr ? 1 : 0);
+    // cr_bw -1: the pivoted elimination for every system (the plain method)
+    k_recompute_body_batch<8><<<(unsigned)nsys, (unsigned)nt, fused_recompute_lds(max_n, cr), s>>>(
+        cases, sys, cr ? cr_max_bw() : -1);
     k_eps_outside_gather_batch<<<(unsigned)((nsys + 255) / 256), 256, 0, s>>>(cases, sys, nsys, outside_out);
     return true;
 }
