@@ -347,6 +347,17 @@ void launch_pending(life_case* c, EarlyBulk& eb) {
     eb.want = false;
 }
 
+// How long the host polls for the device FEM's sums before it waits on the
+// launch's event instead (microseconds; environment LIFE_SUMS_SPIN_US, 0:
+// the event wait alone).
+double sums_spin_us() {
+    static const double v = [] {
+        const char* e = std::getenv("LIFE_SUMS_SPIN_US");
+        return e && *e ? std::atof(e) : 20000.0;
+    }();
+    return v;
+}
+
 // objectKernel with the FEM on the device: per subiteration the host only
 // reads the three residual sums (and the out-of-band flags of epsilon).
 void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
@@ -384,9 +395,12 @@ void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
             so.nflags = g->eps_nsys;
         }
         for (;;) {
+            so.seq = fd.seq += 1.0;
             launch_fem_dynamic(fd.d_bodies, nullptr, fd.nb, fd.lds, redo, g->stream, so);
             LIFE_HIP(hipEventRecord(c->ev_forces, g->stream));
-            LIFE_HIP(hipEventSynchronize(c->ev_forces));
+            // the host polls the sequence word the launch writes last (an
+            // event wait wakes the host tens of microseconds later)
+            fem_wait_sums(so, c->ev_forces, sums_spin_us());
             if (!finish_epsilon(c)) break;
             redo = true;
         }

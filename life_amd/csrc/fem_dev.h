@@ -58,7 +58,10 @@ struct FemDevCase {
     FemDevBody* d_bodies = nullptr;
     std::vector<void*> blocks;     // device allocations
     int* done = nullptr;           // device: bodies finished in the running launch
-    double* sums_h = nullptr;      // page-locked: res, num, den over the bodies, in order
+    // page-locked: res, num, den over the bodies, in order; [3] the sequence
+    // number of the launch that wrote them (written last)
+    double* sums_h = nullptr;
+    double seq = 0.0;              // sequence number of the latest launch
     size_t lds = 0;                // dynamic LDS of the solve kernels
     bool ok() const { return nb > 0 && d_bodies != nullptr; }
 };
@@ -94,15 +97,24 @@ void launch_fem_relax(const FemDevBody* b, const int* list, int nb, double omega
                       hipStream_t st);
 // Where the dynamic launch over bodies 0 .. nb-1 of one case leaves the
 // case's subiteration sums (done null: nowhere).  done: a device int, zero
-// between launches; sums_h: page-locked, 3 doubles; flags (device) ->
-// flags_h (page-locked): nflags ints copied alongside.
+// between launches; sums_h: page-locked, 4 doubles: the sums, then seq,
+// written after them and the flags with a system-scope fence between, so a
+// host that sees seq there sees the rest; flags (device) -> flags_h
+// (page-locked): nflags ints copied alongside.
 struct FemSumsOut {
     int* done = nullptr;
     double* sums_h = nullptr;
+    double seq = 0.0;
     const int* flags = nullptr;
     int* flags_h = nullptr;
     int nflags = 0;
 };
+
+// Wait until the launch with sequence number seq has written its sums
+// (polling the page-locked word, for at most spin_us microseconds), else
+// until ev (recorded after that launch) completes.  Returns with the sums
+// and flags readable.
+void fem_wait_sums(const FemSumsOut& so, hipEvent_t ev, double spin_us);
 
 // femKernel: the dynamic Newmark/Newton solve of every listed body, its new
 // marker positions and velocities, and its subiteration residual terms.

@@ -16,6 +16,8 @@
 //  - dot products: one lane, in index order.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -776,7 +778,8 @@ __global__ __launch_bounds__(kFemThreads) void k_fem_relax(const FemDevBody* __r
 // with device-scope fences on both sides; nothing waits) adds the bodies'
 // subiteration terms out[0..2] in body order, as case_fem does, and writes
 // them and the epsilon systems' out-of-band flags into page-locked host
-// memory, then re-arms the count for the next launch.
+// memory, re-arms the count for the next launch, and last (after a
+// system-scope fence) writes the launch's sequence number beside the sums.
 FDI void sums_by_last(const FemDevBody* __restrict__ bodies, const FemSumsOut& so) {
     __shared__ int last;
     __shared__ double t[3 * kFemThreads];
@@ -796,6 +799,8 @@ FDI void sums_by_last(const FemDevBody* __restrict__ bodies, const FemSumsOut& s
         t[3 * b + 2] = o[2];
     }
     for (int k = threadIdx.x; k < so.nflags; k += blockDim.x) so.flags_h[k] = so.flags[k];
+    // each lane's flag copies reach the host before lane 0 writes seq
+    __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = 0.0, num = 0.0, den = 0.0;
@@ -809,6 +814,8 @@ FDI void sums_by_last(const FemDevBody* __restrict__ bodies, const FemSumsOut& s
         so.sums_h[1] = num;
         so.sums_h[2] = den;
         atomicExch(so.done, 0);
+        __threadfence_system();
+        *(volatile double*)&so.sums_h[3] = so.seq;
     }
 }
 
@@ -1019,6 +1026,22 @@ void fem_dev_create(FemDevCase& fc, const std::vector<const FemBody*>& bodies, c
     LIFE_HIP(hipMemsetAsync(fc.done, 0, sizeof(int), st));
     LIFE_HIP(hipHostMalloc((void**)&fc.sums_h, 4 * sizeof(double), hipHostMallocDefault));
     LIFE_HIP(hipStreamSynchronize(st));
+}
+
+void fem_wait_sums(const FemSumsOut& so, hipEvent_t ev, double spin_us) {
+    const volatile double* w = so.sums_h + 3;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        if (*w == so.seq) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return;
+        }
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        if (us > spin_us) break;
+    }
+    // not seen within the spin: the event (and any error of the launch)
+    LIFE_HIP(hipEventSynchronize(ev));
+    std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 void fem_dev_destroy(FemDevCase& fc) {
