@@ -274,6 +274,15 @@ def bench_serve(args):
 
     stream = cfg.serving_stream(n=args.serve_cases, steps=args.serve_steps)
     wl = serve_workload(args, len(stream))
+    if args.capacity_only:
+        # the engine alone with every case queued at once (for profiling)
+        cap, _ = serve_run([dict(r, arrival=0.0) for r in stream], args.serve_slots, args.serve_budget)
+        print(json.dumps({"metric": "FSI cases/hour (serving engine, all cases queued at once)",
+                          "value": cap["cases_per_hour"], "unit": "cases/hour", "n_gpus": 1,
+                          "steps": args.serve_steps, "warmup": 0, "ms_per_step": cap["ms_per_engine_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic", "config": wl, "serving": {"served_fp32_all_queued_at_once": cap}}))
+        return
     base = args.baseline_cases if args.baseline_cases > 0 else len(stream)
     cmp_ = serving_comparison(stream[:base], args.serve_slots, args.serve_budget, args.accuracy_cases,
                               args.accuracy_steps)
@@ -432,6 +441,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="canopy: leave out the serving leg")
+    ap.add_argument("--capacity-only", action="store_true",
+                    help="serve: the engine alone, every case queued at once (no plain path, no accuracy)")
     ap.add_argument("--serve-cases", type=int, default=256)
     ap.add_argument("--serve-steps", type=int, default=5000)
     ap.add_argument("--serve-slots", type=int, default=128)
