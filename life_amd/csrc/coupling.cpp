@@ -521,12 +521,14 @@ void case_blowup_cadence(life_case* c) {
 
 // The FEM runs on the device for FP32-shifted populations unless the
 // environment sets LIFE_DEVICE_FEM = 0 (or a body does not fit the device
-// FEM's LDS); the plain method always keeps the host FEM, whose
-// transcendental functions match the oracle's bit for bit.
+// FEM's LDS).  The plain method keeps the host FEM, whose transcendental
+// functions match the oracle's bit for bit, unless LIFE_DEVICE_FEM = 1 asks
+// for the device FEM (which then agrees with the oracle to rounding).
 bool device_fem_wanted(int precision) {
-    if (precision != LIFE_FP32_SHIFTED) return false;
     const char* e = std::getenv("LIFE_DEVICE_FEM");
-    return !(e && *e && std::atoi(e) == 0);
+    const bool set = e && *e;
+    if (precision == LIFE_FP32_SHIFTED) return !(set && std::atoi(e) == 0);
+    return set && std::atoi(e) != 0;
 }
 
 // CUs kept free of the bulk stream-collide for a device-FEM coupling chain
