@@ -124,7 +124,8 @@ void fem_wait_sums(const FemSumsOut& so, hipEvent_t ev, double spin_us);
 // before reading them).
 void launch_fem_dynamic(const FemDevBody* b, const int* list, int nb, size_t lds, bool redo, hipStream_t st,
                         const FemSumsOut& so = FemSumsOut());
-// out[0..2] of the listed bodies packed into outs (3 per body).
+// out[0..2] of the listed bodies packed into outs (3 per body; outs may be
+// page-locked host memory, readable after the stream is synchronised).
 void launch_fem_outs(const FemDevBody* b, const int* list, int nb, double* outs, hipStream_t st);
 
 }  // namespace life

@@ -176,7 +176,9 @@ void launch_find_support_batch(const CaseDev* cases, const int* list, int n, int
                                hipStream_t s);
 void launch_compute_ds_batch(const CaseDev* cases, const int* list, int n, int max_bodies, int flexible_only,
                              hipStream_t s);
-// outside_out[q] = 1 for a listed system that is not banded (no solve done);
+// outside_out[q] = 1 for a listed system that is not banded (no solve done;
+// outside_out may be page-locked host memory, readable after the stream is
+// synchronised);
 // cr as for launch_eps_band
 void launch_eps_band_batch(const CaseDev* cases, const CaseSys* sys, int nsys, int max_n, int* outside_out,
                            bool cr, hipStream_t s);

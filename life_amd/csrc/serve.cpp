@@ -583,9 +583,9 @@ void fem_solve_device(life_server* s, const std::vector<int>& active, const std:
     // the device has room beside the FEM's few workgroups: the next step's
     // bulk stream-collide starts now
     if (s->early == 2) launch_bulk_early(s);
-    launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.d, st);
+    // each body's terms go straight into the page-locked host array
+    launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.h, st);
     LIFE_HIP(hipGetLastError());
-    s->fem_out.download(3 * (size_t)nfb, st);
     LIFE_HIP(hipStreamSynchronize(st));
     std::vector<int> redo;  // slots whose epsilon changed
     for (size_t k = 0; k < sysl.size(); k++)
@@ -601,9 +601,8 @@ void fem_solve_device(life_server* s, const std::vector<int>& active, const std:
             for (int b = 0; b < s->slots[q].c->fdev.nb; b++) s->fem_list2.h[n++] = s->slots[q].fem_off + b;
         s->fem_list2.upload(n, st);
         launch_fem_dynamic(s->fem_bodies.d, s->fem_list2.d, n, s->fem_lds, true, st);
-        launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.d, st);
+        launch_fem_outs(s->fem_bodies.d, s->fem_list.d, nfb, s->fem_out.h, st);
         LIFE_HIP(hipGetLastError());
-        s->fem_out.download(3 * (size_t)nfb, st);
         LIFE_HIP(hipStreamSynchronize(st));
     }
     s->times.lap(st, 4);
@@ -688,16 +687,17 @@ void coupling_phase(life_server* s) {
             // FP32-shifted cases solve their banded systems by cyclic
             // reduction, as life_case does
             const bool cr = s->cfg.precision == LIFE_FP32_SHIFTED;
+            // the out-of-band flags are gathered straight into the
+            // page-locked host array, read after the next synchronisation
             const bool fused = !any_uni && !sysl.empty() &&
                                launch_recompute_fused_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n,
-                                                            s->outside.d, cr, st);
+                                                            s->outside.h, cr, st);
             if (!fused) {
                 launch_find_support_batch(s->cases.d, s->dyn.d, na, s->max_markers, 1, st);
                 launch_compute_ds_batch(s->cases.d, s->dyn.d, na, s->max_bodies, 1, st);
                 if (!sysl.empty())
-                    launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.d, cr, st);
+                    launch_eps_band_batch(s->cases.d, s->sys.d, (int)sysl.size(), max_n, s->outside.h, cr, st);
             }
-            if (!sysl.empty()) s->outside.download(sysl.size(), st);
         }
         // ibmKernelInterp: every case with bodies on the first pass, then
         // the cases still subiterating
