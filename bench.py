@@ -538,7 +538,15 @@ def main():
         w64 = time.perf_counter() - t1
         _, _, u64, _ = ref.fields()
         accuracy = {"u_l2_rel_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)), "steps": t_end}
-        del u32, u64
+        # the yardstick: FP64 against itself with the inflow perturbed by
+        # 1e-12 (relative), at the same steps (once the flow reaches the
+        # filaments, their motion amplifies any difference)
+        print("bench: FP64 with the inflow perturbed by 1e-12", file=sys.stderr, flush=True)
+        per = Case(p.replace(uxInlet_p=p.uxInlet_p * (1.0 + 1e-12)), geom, precision=lib.LIFE_FP64)
+        per.step(t_end)
+        _, _, up, _ = per.fields()
+        accuracy["fp64_perturbed_1e-12_vs_fp64"] = float(np.linalg.norm(up - u64) / np.linalg.norm(u64))
+        del u32, u64, up
         k64_s = k64 / 1e3 / args.steps
         fp64 = {"mlups": N * args.steps / w64 / 1e6, "ms_per_step": w64 * 1e3 / args.steps,
                 "roofline": {"bound": "hbm", "kernel": "k_push64", "kernel_ms": k64_s * 1e3,
@@ -548,12 +556,16 @@ def main():
             print("bench: FP32 and FP64 to step %d" % args.horizon, file=sys.stderr, flush=True)
             case.step(args.horizon - case.t)
             ref.step(args.horizon - ref.t)
+            per.step(args.horizon - per.t)
             _, _, a, _ = case.fields()
             _, _, b, _ = ref.fields()
+            _, _, c, _ = per.fields()
             accuracy["horizon"] = {"steps": args.horizon,
                                    "u_l2_rel_vs_fp64": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                                   "fp64_perturbed_1e-12_vs_fp64": float(np.linalg.norm(c - b) / np.linalg.norm(b)),
                                    "subiterations_last_step": case.coupling()["subIt"]}
-            del a, b
+            del a, b, c
+        per.close()
         ref.close()
     case.close()
 
