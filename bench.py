@@ -422,6 +422,36 @@ def bench_config(args):
     print(json.dumps(out))
 
 
+def bench_schafer_turek(args):
+    """The Schaefer-Turek cylinder benchmarks on the reference's Cylinder
+    example by resolution factor (D = 10 r lattice units): 2D-1 (Re 20,
+    30 s) at r = 2 and 4, 2D-2 (Re 100, 20 s) at r = 2, 4 and 8 on the plain
+    FP64 path, and 2D-2 at r = 4 on the FP32-shifted path; each coefficient
+    beside its published bounds.  The headline number is the MLUPS of the
+    largest FP64 run (1761 x 329, 1.28 M steps)."""
+    from life_amd import lib
+    from life_amd.config import SCHAFER_TUREK_BOUNDS
+    from life_amd.validation import schafer_turek_run
+
+    runs = []
+    for test, r, secs, prec in (("2D-1", 2, 30.0, lib.LIFE_FP64), ("2D-1", 4, 30.0, lib.LIFE_FP64),
+                                ("2D-2", 2, 20.0, lib.LIFE_FP64), ("2D-2", 4, 20.0, lib.LIFE_FP64),
+                                ("2D-2", 8, 20.0, lib.LIFE_FP64), ("2D-2", 4, 20.0, lib.LIFE_FP32_SHIFTED)):
+        out = schafer_turek_run(test, r, seconds=secs, precision=prec)
+        out["precision"] = "f64" if prec == lib.LIFE_FP64 else "f32_shifted"
+        out["mlups"] = out["Nx"] * out["Ny"] * out["steps"] / out["wall_s"] / 1e6
+        runs.append(out)
+        print("bench: schafer_turek %s" % json.dumps(out), file=sys.stderr, flush=True)
+    big = runs[4]
+    out = {"metric": "D2Q9 MLUPS (plain FP64, full step incl. coupling)", "value": big["mlups"], "unit": "MLUPS",
+           "n_gpus": 1, "steps": big["steps"], "warmup": 0, "ms_per_step": big["wall_s"] * 1e3 / big["steps"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "schafer_turek", "Nx": big["Nx"], "Ny": big["Ny"],
+                      "description": "Schaefer-Turek 2D-1/2D-2 on the reference's Cylinder example, r = 2, 4, 8"},
+           "results": {"published_bounds": SCHAFER_TUREK_BOUNDS, "runs": runs}}
+    print(json.dumps(out))
+
+
 def main():
     if "--calibrate-copy" in sys.argv:
         calibrate_copy(1 << 30, 5)
@@ -468,6 +498,9 @@ def main():
         return
     if args.workload in CONFIGS:
         bench_config(args)
+        return
+    if args.workload == "schafer_turek":
+        bench_schafer_turek(args)
         return
 
     import numpy as np
