@@ -35,10 +35,8 @@ hipError_t set_coupling_priority(int level) {
 // IBMNodeClass::findSupport (IBMNode.cpp:139-179): lattice sites within 1.5
 // spacings of the marker in both directions, scanned i-major over the 5x5
 // block around the nearest site, kept in scan order (at most 9).
-__device__ inline void find_support_one(const MarkerDev& m, const GridScalars& g, int flexible_only, int k) {
-    if (flexible_only && m.body_flex[m.body[k]] != 0) return;
+__device__ inline void find_support_xy(const MarkerDev& m, const GridScalars& g, int k, double x, double y) {
     const double Dx = g.Dx;
-    const double x = m.pos[2 * k], y = m.pos[2 * k + 1];
     const int inear = (int)round(x / Dx), jnear = (int)round(y / Dx);
     int n = 0;
     for (int i = inear - 2; i <= inear + 2; i++) {
@@ -67,6 +65,11 @@ __device__ inline void find_support_one(const MarkerDev& m, const GridScalars& g
     m.supp_cnt[k] = n;
 }
 
+__device__ inline void find_support_one(const MarkerDev& m, const GridScalars& g, int flexible_only, int k) {
+    if (flexible_only && m.body_flex[m.body[k]] != 0) return;
+    find_support_xy(m, g, k, m.pos[2 * k], m.pos[2 * k + 1]);
+}
+
 __global__ void k_find_support(MarkerDev m, GridScalars g, int flexible_only) {
     coupling_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,15 +87,19 @@ __global__ void k_find_support_batch(const CaseDev* __restrict__ cases, const in
 // to the nearest other marker of the same body, starting from 10.  One
 // workgroup per body; the body's positions are staged in LDS when they fit.
 constexpr int kDsLdsMarkers = 2048;
-__device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g, int flexible_only, int b,
-                                       double* spos) {
-    if (flexible_only && m.body_flex[b] != 0) return;
-    const int first = m.body_first[b], cnt = m.body_n[b];
+
+// The positions of body markers [first, first + cnt) staged in LDS (spos)
+// when they fit, else read from device memory; all lanes of the workgroup.
+__device__ inline const double* stage_body_positions(const MarkerDev& m, int first, int cnt, double* spos) {
     const bool lds = cnt <= kDsLdsMarkers;
     if (lds)
         for (int t = threadIdx.x; t < 2 * cnt; t += blockDim.x) spos[t] = m.pos[2 * first + t];
     __syncthreads();
-    const double* P = lds ? spos : m.pos + 2 * first;
+    return lds ? spos : m.pos + 2 * first;
+}
+
+__device__ inline void ds_from_positions(const MarkerDev& m, const GridScalars& g, int first, int cnt,
+                                         const double* P) {
     const double2* P2 = reinterpret_cast<const double2*>(P);
     // sqrt(.)/Dx is monotone, so the minimum of sqrt(d.d)/Dx over the other
     // markers is sqrt(min d.d)/Dx: one square root per marker, same value.
@@ -131,6 +138,13 @@ __device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g,
     }
 }
 
+__device__ inline void compute_ds_body(const MarkerDev& m, const GridScalars& g, int flexible_only, int b,
+                                       double* spos) {
+    if (flexible_only && m.body_flex[b] != 0) return;
+    const int first = m.body_first[b], cnt = m.body_n[b];
+    ds_from_positions(m, g, first, cnt, stage_body_positions(m, first, cnt, spos));
+}
+
 __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
     coupling_priority();
     extern __shared__ double spos[];
@@ -140,13 +154,18 @@ __global__ void k_compute_ds(MarkerDev m, GridScalars g, int flexible_only) {
 // Supports and spacing of one body per workgroup in one launch: the two
 // are independent (spacing reads only positions), so the coupling chain
 // has one kernel fewer to wait for.
+// The body's positions are read once (into LDS), and its kind once per
+// workgroup, so beside a bandwidth-bound kernel the chain of dependent
+// device-memory reads is two long (body table, positions).
 __global__ void k_support_ds(MarkerDev m, GridScalars g, int flexible_only) {
     coupling_priority();
     extern __shared__ double spos[];
     const int b = blockIdx.x;
     const int first = m.body_first[b], cnt = m.body_n[b];
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) find_support_one(m, g, flexible_only, first + k);
-    compute_ds_body(m, g, flexible_only, b, spos);
+    if (flexible_only && m.body_flex[b] != 0) return;  // the whole workgroup
+    const double* P = stage_body_positions(m, first, cnt, spos);
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) find_support_xy(m, g, first + k, P[2 * k], P[2 * k + 1]);
+    ds_from_positions(m, g, first, cnt, P);
 }
 
 __global__ void k_compute_ds_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list, int flexible_only) {
@@ -194,14 +213,25 @@ __device__ inline void eps_band_entry(const MarkerDev& m, const GridScalars& g, 
     if (q <= 2 * KB && c >= 0 && c < n) {
         const int mi = first + r, mj = first + c;
         const double Dx = g.Dx;
-        const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
+        // every read issued before the arithmetic: the support slots past
+        // the count hold zeros and are read but not used
+        const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx, dsj = m.ds[mj];
         const int cnt = m.supp_cnt[mi];
-        for (int s = 0; s < cnt; s++) {
-            const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
-            const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
-            a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
+        int sij[2 * kSupp];
+        double sd[kSupp];
+#pragma unroll
+        for (int s = 0; s < 2 * kSupp; s++) sij[s] = m.supp_ij[mi * kSupp * 2 + s];
+#pragma unroll
+        for (int s = 0; s < kSupp; s++) sd[s] = m.supp_delta[mi * kSupp + s];
+#pragma unroll
+        for (int s = 0; s < kSupp; s++) {
+            if (s < cnt) {
+                const double dx = fabs(xj - sij[2 * s]);
+                const double dy = fabs(yj - sij[2 * s + 1]);
+                a += sd[s] * (dirac64(dx) * dirac64(dy));
+            }
         }
-        a = a * m.ds[mj];
+        a = a * dsj;
     }
     band[t] = a;
 }
@@ -913,9 +943,11 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
         *bw_sh = 0;
     }
     for (int k = tid; k < n; k += nt) {
-        find_support_one(m, g, 1, first + k);
-        P[2 * k] = m.pos[2 * (first + k)];
-        P[2 * k + 1] = m.pos[2 * (first + k) + 1];
+        // the position and the body kind are read side by side
+        const double x = m.pos[2 * (first + k)], y = m.pos[2 * (first + k) + 1];
+        if (m.body_flex[m.body[first + k]] == 0) find_support_xy(m, g, first + k, x, y);
+        P[2 * k] = x;
+        P[2 * k + 1] = y;
     }
     __syncthreads();
     // computeDs (IBMNode.cpp:182-204), as compute_ds_body

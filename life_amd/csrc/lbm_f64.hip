@@ -77,9 +77,9 @@ __global__ __launch_bounds__(256) void k_push64(const double* __restrict__ src, 
 // values reach the same addresses as k_push64, with aligned wavefront stores.
 // SP/DP: double pointers, generic or in the global address space.  SITEF:
 // force_xy per site (g.fprev / g.fcur), else the uniform g.Fxm / g.Fx0.
-template <bool CM, bool SITEF, typename SP, typename DP>
+template <bool CM, bool SITEF, int BS, bool NTS, typename SP, typename DP>
 __device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restrict__ dst, const GridScalars& g,
-                                                 long long base, int t, PushStage<double>& st) {
+                                                 long long base, int t, PushStage<double, BS>& st) {
     const long long id = base + t;
     const bool valid = id < g.N;
     const long long idc = valid ? id : 0;
@@ -102,15 +102,15 @@ __device__ __forceinline__ void push64_lds_block(SP __restrict__ src, DP __restr
         moments64(fs, g.Fxm, g.Fym, rho, ux, uy);
         collide64<CM>(g, fs, rho, ux, uy, g.Fx0, g.Fy0, fo);
     }
-    push_store_realigned<double, true>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
+    push_store_realigned<double, NTS, BS>(dst, S, g.Nx, Ny, g.N, base, t, idc, i, j, fo, valid, st);
 }
 
-template <bool CM, bool SITEF>
-__global__ __launch_bounds__(kPushBlock) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
-                                                           GridScalars g, unsigned nblocks) {
-    __shared__ PushStage<double> st;
-    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * kPushBlock;
-    push64_lds_block<CM, SITEF>(src, dst, g, base, threadIdx.x, st);
+template <bool CM, bool SITEF, int BS = kPushBlock, bool NTS = true>
+__global__ __launch_bounds__(BS) void k_push64_lds(const double* __restrict__ src, double* __restrict__ dst,
+                                                   GridScalars g, unsigned nblocks) {
+    __shared__ PushStage<double, BS> st;
+    const long long base = (long long)xcd_block(blockIdx.x, nblocks) * BS;
+    push64_lds_block<CM, SITEF, BS, NTS>(src, dst, g, base, threadIdx.x, st);
 }
 
 // Push of one site with periodic wrap (Grid.cpp:226-244).
@@ -133,7 +133,7 @@ __device__ __forceinline__ void push_site64(double* __restrict__ dst, const Grid
 template <bool CM>
 __global__ __launch_bounds__(kServeBlock64) void k_push64_batch(const CaseDev* __restrict__ cases,
                                                              const CaseBlock* __restrict__ blocks) {
-    __shared__ PushStage<double> st;
+    __shared__ PushStage<double, kServeBlock64> st;
     // neighbouring blocks of a case on one XCD (the L2 that merges their
     // shared edge lines)
     const CaseBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
@@ -150,7 +150,8 @@ __global__ __launch_bounds__(kServeBlock64) void k_push64_batch(const CaseDev* _
         collide64<CM>(c.g, fs, c.rho0[id], c.u0[2 * id], c.u0[2 * id + 1], Fx, Fy, fo);
         push_site64(dst, c.g, id, fo);
     } else {
-        push64_lds_block<CM, false>((gptr<const double>)c.src, (gptr<double>)c.dst, c.g, base, threadIdx.x, st);
+        push64_lds_block<CM, false, kServeBlock64, true>((gptr<const double>)c.src, (gptr<double>)c.dst, c.g, base,
+                                                         threadIdx.x, st);
     }
 }
 
@@ -195,20 +196,52 @@ __global__ __launch_bounds__(256) void k_support_push64_batch(const CaseDev* __r
                             c.sc.uniq[k]);
 }
 
+// Bulk kernel configuration, read at every launch (tuning sweeps):
+// LIFE_PUSH64 = scalar selects k_push64; LIFE_PUSH64_BS = 256 | 512 | 1024
+// lanes per block of k_push64_lds (512 by default: on the 8192 x 4096
+// lattice a step took 0.967-1.023 ms against 1.044-1.048 ms at 256 and
+// 0.970-0.982 at 1024); LIFE_PUSH64_NT = 0 | 1 streaming stores (off:
+// 13-16 % slower).
+struct Push64Config {
+    bool lds = true;
+    int bs = 512;
+    bool nt = true;
+    Push64Config() {
+        if (const char* e = std::getenv("LIFE_PUSH64")) lds = std::strcmp(e, "scalar") != 0;
+        if (const char* e = std::getenv("LIFE_PUSH64_BS")) {
+            const int v = std::atoi(e);
+            bs = v == 256 || v == 1024 ? v : 512;
+        }
+        if (const char* e = std::getenv("LIFE_PUSH64_NT")) nt = std::atoi(e) != 0;
+    }
+};
+
+template <bool CM, bool SITEF>
+static void launch_push64_lds(const double* src, double* dst, const GridScalars& g, const Push64Config& c,
+                              hipStream_t s) {
+    const unsigned nb = (unsigned)((g.N + c.bs - 1) / c.bs);
+    if (c.bs == 512) {
+        if (c.nt) k_push64_lds<CM, SITEF, 512, true><<<nb, 512, 0, s>>>(src, dst, g, nb);
+        else k_push64_lds<CM, SITEF, 512, false><<<nb, 512, 0, s>>>(src, dst, g, nb);
+    } else if (c.bs == 1024) {
+        if (c.nt) k_push64_lds<CM, SITEF, 1024, true><<<nb, 1024, 0, s>>>(src, dst, g, nb);
+        else k_push64_lds<CM, SITEF, 1024, false><<<nb, 1024, 0, s>>>(src, dst, g, nb);
+    } else {
+        if (c.nt) k_push64_lds<CM, SITEF, kPushBlock, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+        else k_push64_lds<CM, SITEF, kPushBlock, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+    }
+}
+
 void launch_push64(const double* src, double* dst, const GridScalars& g, const double* rho0, const double* u0,
                    const double* fibm, bool explicit_state, hipStream_t s) {
-    static const bool lds = [] {
-        const char* e = std::getenv("LIFE_PUSH64");
-        return !(e && std::strcmp(e, "scalar") == 0);
-    }();
-    if (!explicit_state && lds) {
-        const unsigned nb = (unsigned)((g.N + kPushBlock - 1) / kPushBlock);
+    const Push64Config c;
+    if (!explicit_state && c.lds) {
         if (g.fcur) {
-            if (g.cm) k_push64_lds<true, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
-            else k_push64_lds<false, true><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+            if (g.cm) launch_push64_lds<true, true>(src, dst, g, c, s);
+            else launch_push64_lds<false, true>(src, dst, g, c, s);
         } else {
-            if (g.cm) k_push64_lds<true, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
-            else k_push64_lds<false, false><<<nb, kPushBlock, 0, s>>>(src, dst, g, nb);
+            if (g.cm) launch_push64_lds<true, false>(src, dst, g, c, s);
+            else launch_push64_lds<false, false>(src, dst, g, c, s);
         }
         return;
     }
