@@ -4,8 +4,10 @@ Workload (default): the canopy case -- D2Q9 BGK on an 8192 x 4096 lattice
 with FP32-shifted populations, uniform inlet, convective outlet, no-slip
 bottom, free-slip top, and 64 clamped flexible filaments (length 200, 201
 markers and 50 corotational beam elements each) coupled through the
-implicit immersed boundary with FP64 FEM on the host.  A step is one full
-time step (lattice step + coupling subiterations + spread).
+implicit immersed boundary, with the FP64 FEM on the device (one workgroup
+per filament; LIFE_DEVICE_FEM=0 selects the host FEM, which the FP64
+populations use by default).  A step is one full time step (lattice step +
+coupling subiterations + spread).
 
 Prints one JSON line: MLUPS of the whole step (lattice sites x steps /
 wall time), the roofline of the fused stream-collide kernel (72 algorithmic
@@ -486,7 +488,7 @@ def main():
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build runs each case on a single MI355X "
+        print("bench.py measures one GPU only: the library runs each case on a single MI355X "
               "(multi-GPU sharding is a design note, not an implementation).", file=sys.stderr)
         sys.exit(2)
     # the CPU baseline's OpenMP threads stay on neighbouring cores (read by

@@ -59,8 +59,9 @@ def test_canopy_full_size_fp64_matches_oracle(oracle, gpu_lib):
 
 def test_canopy_full_size_fp32_accuracy(gpu_lib):
     """FP32-shifted against FP64 on the canopy after the inlet ramp (2000
-    steps) and 500 steps of loaded flow: velocity L2 <= 1e-5 (the north-star
-    bar); the FP64 run is the oracle's (previous test)."""
+    steps) and 500 steps of loaded flow: velocity L2 <= 1e-5 (the stated
+    bar of the shifted-FP32 form); the FP64 run is the oracle's (previous
+    test)."""
     p, g = cfg.canopy()
     steps = 2500
     out = {}
