@@ -118,6 +118,12 @@ struct life_case {
     // calls case_sync_host.
     life::FemDevCase fdev;
     bool host_stale = false;
+    // Device FEM: the spread is queued right behind the FEM of a
+    // subiteration predicted to be the step's last (as many as the previous
+    // step took), before the host reads the residual (LIFE_SPEC_SPREAD = 0
+    // turns this off).
+    bool spec_spread = true;
+    int prev_subits = 0;
 
     life::MarkerView view(const life::Body& b) {
         return life::MarkerView{pos.data(), vel.data(), force.data(), eps.data(), b.first, b.n};

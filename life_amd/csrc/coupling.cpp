@@ -366,7 +366,7 @@ void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
     PhaseTimes& T = *c->timing;
     c->subIt = 0;
     c->forces_on_host = false;
-    bool more = false;
+    bool more = false, spread_done = false;
     do {
         // recomputeObjectVals: predictor or relaxation moves the markers
         if (c->subIt == 0) {
@@ -394,13 +394,24 @@ void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
             so.flags_h = g->eps_outside_h;
             so.nflags = g->eps_nsys;
         }
+        // The final spread (ObjectsClass::ibmKernelSpread) takes the forces
+        // of the last interpolation and the supports found before the last
+        // FEM solve, so it does not wait for that solve's residual: when
+        // this subiteration is predicted to be the last, it is queued
+        // behind the FEM and runs while the host reads the residual.  If
+        // another subiteration follows after all, its spread clears every
+        // site of this one and writes its own, so the result is the same.
+        const bool spec = c->spec_spread && c->subIt + 1 >= c->prev_subits;
         for (;;) {
             so.seq = fd.seq += 1.0;
             launch_fem_dynamic(fd.d_bodies, nullptr, fd.nb, fd.lds, redo, g->stream, so);
             LIFE_HIP(hipEventRecord(c->ev_forces, g->stream));
+            if (spec) ibm_spread(g);
+            spread_done = spec;
             // the host polls the sequence word the launch writes last (an
             // event wait wakes the host tens of microseconds later)
             fem_wait_sums(so, c->ev_forces, sums_spin_us());
+            // a redo changes epsilon: the spread is queued again behind it
             if (!finish_epsilon(c)) break;
             redo = true;
         }
@@ -413,7 +424,8 @@ void object_kernel_dev(life_case* c, EarlyBulk& eb, int mode) {
         c->subIt++;
         more = case_subiterate(c);
     } while (more);
-    ibm_spread(g);
+    if (!spread_done) ibm_spread(g);
+    c->prev_subits = c->subIt;
     T.lap(g, 4);
     c->host_stale = true;
     if (c->subIt == kMaxSubIt) throw std::runtime_error("Subiteration scheme hit 20 iterations");
@@ -499,6 +511,7 @@ void case_step(life_case* c, hipEvent_t ev_begin, hipEvent_t ev_end, bool more, 
     T.start(c->g);
     c->t++;
     lbm_step(c->g, c->t, ev_begin, ev_end);
+    if (T.on) lbm_join(c->g);  // the lattice phase ends with the step's tail, wherever it ran
     T.lap(c->g, 0);
     // without immersed bodies there is nothing to overlap the early
     // stream-collide with: the plain order avoids the cross-stream waits
@@ -533,8 +546,15 @@ bool device_fem_wanted(int precision) {
 
 // CUs kept free of the bulk stream-collide for a device-FEM coupling chain
 // (loaded canopy, 8192x4096, two subiterations per step: 0.69 ms/step with
-// no reserve, 0.60 with 16 or 32 CUs).
+// no reserve, 0.60 with 16 or 32 CUs), where the XCD split below is not
+// available.
 constexpr int kDeviceFemCus = 32;
+// The device-FEM coupling chain on two whole XCDs and the early bulk
+// stream-collide on the other six (loaded canopy: 0.543-0.545 ms/step
+// against 0.559-0.560 with the 32-CU reserve, which lets the chain's
+// kernels share CUs with the bulk kernel; one or three XCDs for the chain:
+// 0.677 and 0.571).
+constexpr int kDeviceFemXcds = 2;
 
 void setup_device_fem(life_case* c) {
     std::vector<const FemBody*> fb;
@@ -550,8 +570,8 @@ void setup_device_fem(life_case* c) {
     life_grid* g = c->g;
     fem_dev_create(c->fdev, fb, first, nm, g->md.pos, g->md.vel, g->md.force, g->md.eps, s, g->stream);
     // With the FEM on the device the whole coupling chain competes with the
-    // early bulk stream-collide for CUs: keep kDeviceFemCus CUs free of it.
-    grid_reserve_coupling_cus(g, kDeviceFemCus);
+    // early bulk stream-collide for CUs: the two get XCDs of their own.
+    grid_split_coupling(g, kDeviceFemXcds, kDeviceFemCus);
 }
 
 }  // namespace
@@ -606,6 +626,7 @@ int life_case_create(const life_params* p, const char* geometry, int precision, 
             LIFE_HIP(hipStreamSynchronize(c->g->stream));
             check_flags(c->g);
             if (c->hasFlex && device_fem_wanted(precision)) setup_device_fem(c);
+            if (const char* e = std::getenv("LIFE_SPEC_SPREAD")) c->spec_spread = std::atoi(e) != 0;
         }
     });
     if (rc) {

@@ -334,6 +334,8 @@ void lbm_bulk_early(life_grid* g, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (!g->tail_pending) {
         LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
         LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_ready, 0));
+    } else if (g->tail_stream != g->stream2) {
+        LIFE_HIP(hipStreamWaitEvent(g->stream2, g->ev_tail, 0));
     }
     hipStream_t s = g->stream2;
     if (ev_begin) LIFE_HIP(hipEventRecord(ev_begin, s));
@@ -359,6 +361,7 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const bool ex = g->explicit_state;
     hipStream_t s = g->stream;
     const bool early = g->bulk_pending;
+    bool off = early;  // the step runs on a stream other than the grid's
     if (early) {
         // the rest of the step follows the bulk kernel on stream2, after
         // the grid stream's work so far (the spread it reads)
@@ -368,6 +371,14 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
         g->bulk_pending = false;
     } else {
         lbm_join(g);
+        if (g->stream_pool) {
+            // the grid's stream holds part of the device: the whole step
+            // runs on the grid's own stream, after that stream's work so far
+            LIFE_HIP(hipEventRecord(g->ev_ready, g->stream));
+            LIFE_HIP(hipStreamWaitEvent(g->stream_pool, g->ev_ready, 0));
+            s = g->stream_pool;
+            off = true;
+        }
     }
     if (g->prec == kFP64) {
         const double* fs = static_cast<const double*>(g->f[src]);
@@ -393,9 +404,10 @@ void lbm_step(life_grid* g, int t, hipEvent_t ev_begin, hipEvent_t ev_end) {
         launch_bc<float>(fs, fd, g->g, a, g->fibm, g->rho0, g->u0, ex, s);
     }
     LIFE_HIP(hipGetLastError());
-    if (early) {
+    if (off) {
         LIFE_HIP(hipEventRecord(g->ev_tail, s));
         g->tail_pending = true;
+        g->tail_stream = s;
     }
     g->cur = dst;
     g->explicit_state = false;
@@ -660,6 +672,48 @@ void grid_reserve_coupling_cus(life_grid* g, int cus) {
     g->stream2 = s;
 }
 
+// A stream on the device's first `xcds` XCDs (inside) or on the others,
+// made once per device and kept with the CU-masked streams (keys below
+// zero).  The CUs of XCD x are x*G .. x*G+G-1 (G = CUs per XCD).  Only
+// whole XCDs: the workgroups of a launch are dealt to the XCDs in turn, so
+// an XCD with part of its CUs enabled holds up every launch (loaded canopy,
+// coupling on 48 or 56 CUs: 0.66 ms/step; on 64, two whole XCDs: 0.544).
+// Null where the CUs do not split into eight equal XCDs.
+static hipStream_t xcd_group_stream(int device, int xcds, bool inside) {
+    MaskedStreams& ms = masked_streams();
+    std::lock_guard<std::mutex> lock(ms.mu);
+    const std::pair<int, int> key{device, inside ? -1 - xcds : -101 - xcds};
+    auto it = ms.streams.find(key);
+    if (it != ms.streams.end()) return it->second;
+    int ncu = 0;
+    LIFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    if (ncu % 8 != 0 || xcds <= 0 || xcds >= 8) return nullptr;
+    const int G = ncu / 8;
+    std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+    for (int c = 0; c < ncu; c++)
+        if ((c < xcds * G) == inside) mask[(size_t)c / 32] |= 1u << (c % 32);
+    hipStream_t st = nullptr;
+    LIFE_HIP(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
+    ms.streams[key] = st;
+    return st;
+}
+
+void grid_split_coupling(life_grid* g, int xcds, int fallback_cus) {
+    if (std::getenv("LIFE_COUPLING_CUS")) return;  // the environment's choice stands
+    if (const char* e = std::getenv("LIFE_COUPLING_XCDS")) xcds = std::atoi(e);
+    hipStream_t cs = xcds > 0 ? xcd_group_stream(g->device, xcds, true) : nullptr;
+    hipStream_t bs = cs ? xcd_group_stream(g->device, xcds, false) : nullptr;
+    if (!cs || !bs) {
+        grid_reserve_coupling_cus(g, fallback_cus);
+        return;
+    }
+    LIFE_HIP(hipStreamSynchronize(g->stream));
+    LIFE_HIP(hipStreamSynchronize(g->stream2));
+    if (!g->stream_pool) g->stream_pool = g->stream;
+    g->stream = cs;
+    g->stream2 = bs;
+}
+
 }  // namespace life
 
 using namespace life;
@@ -879,6 +933,7 @@ int life_grid_destroy(life_grid* g) {
     const int rc = guarded([&] {
         (void)hipSetDevice(g->device);
         if (g->stream) (void)hipStreamSynchronize(g->stream);
+        if (g->stream_pool) (void)hipStreamSynchronize(g->stream_pool);
         if (g->ev_bulk) (void)hipEventSynchronize(g->ev_bulk);  // this grid's last bulk kernel
         if (g->ev_tail) (void)hipEventSynchronize(g->ev_tail);  // and step tail on stream2
         free_markers(g);
@@ -894,7 +949,9 @@ int life_grid_destroy(life_grid* g) {
         if (g->ev_blow) (void)hipEventDestroy(g->ev_blow);
         for (hipEvent_t e : {g->ev_a, g->ev_b, g->ev_k0, g->ev_k1, g->ev_ready, g->ev_bulk, g->ev_tail})
             if (e) (void)hipEventDestroy(e);
-        if (g->stream) give_grid_stream(g->device, g->stream);
+        // the coupling stream of a split grid is the device's, not the grid's
+        if (g->stream_pool) give_grid_stream(g->device, g->stream_pool);
+        else if (g->stream) give_grid_stream(g->device, g->stream);
         delete g;
     });
     handle_release();

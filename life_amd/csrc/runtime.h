@@ -109,6 +109,12 @@ struct life_grid {
     // early bulk stream-collide of the next step on a second stream (one
     // CU-masked stream per device, shared by all grids; not owned)
     hipStream_t stream2 = nullptr;
+    // With the coupling split (grid_split_coupling) `stream` is the
+    // device's coupling stream and this is the grid's own stream from the
+    // pool (the whole device), on which the plain-order lattice step runs;
+    // given back at destroy
+    hipStream_t stream_pool = nullptr;
+    hipStream_t tail_stream = nullptr;  // where the pending step tail runs
     hipEvent_t ev_ready = nullptr, ev_bulk = nullptr, ev_tail = nullptr;
     bool bulk_pending = false;  // a bulk kernel of the next step is queued on stream2
     bool tail_pending = false;  // the last step's tail ran on stream2 (stream not yet joined)
@@ -153,6 +159,15 @@ hipStream_t masked_stream(int device, int reserve_cus);
 // The grid's bulk stream-collide keeps cus CUs free for the coupling kernels
 // (unless the environment sets LIFE_COUPLING_CUS); before any step.
 void grid_reserve_coupling_cus(life_grid* g, int cus);
+// The grid's stream becomes a stream on the device's first `xcds` XCDs, for
+// the coupling chain, and its second stream (the early bulk
+// stream-collide and the step tail) one on the other XCDs; a plain-order
+// lattice step runs on the grid's own stream, on the whole device.  Falls
+// back to grid_reserve_coupling_cus(g, fallback_cus) where the CUs do not
+// split into eight equal XCDs.  Environment: LIFE_COUPLING_XCDS overrides
+// xcds (0: the reserve instead); LIFE_COUPLING_CUS leaves the streams as
+// they are.  Before any step.
+void grid_split_coupling(life_grid* g, int xcds, int fallback_cus);
 // Count of live grids and servers; the last release frees every pooled HIP
 // resource (idle streams, kept device/pinned blocks, CU-masked streams).
 void handle_acquire();

@@ -158,3 +158,23 @@ def test_served_device_fem_equals_case(monkeypatch, gpu_lib):
         assert np.array_equal(r2, rho) and np.array_equal(u2, u) and np.array_equal(p2, pos)
         assert np.array_equal(np.array(info["total_force"]), force)
     srv.close()
+
+
+@pytest.mark.parametrize("name", ["canopy", "flag", "fsi"])
+def test_spread_behind_fem_is_neutral(monkeypatch, gpu_lib, name):
+    """The final spread queued behind the FEM of a subiteration predicted to
+    be the step's last (the default) and the spread queued after the host has
+    read the residual (LIFE_SPEC_SPREAD=0) give the same trajectory bit for
+    bit.  Where a step takes more subiterations than the one before, the
+    queued spread is redone by the next subiteration's."""
+    build, steps = CASES[name]
+    pg = build()
+    monkeypatch.setenv("LIFE_SPEC_SPREAD", "0")
+    a = run(pg, steps, monkeypatch, True)
+    monkeypatch.setenv("LIFE_SPEC_SPREAD", "1")
+    b = run(pg, steps, monkeypatch, True)
+    grew = sum(1 for k in range(1, steps) if b["subits"][k] > b["subits"][k - 1])
+    print("%s: subiterations %s (%d steps with more than the step before)" % (name, b["subits"], grew), flush=True)
+    assert a["subits"] == b["subits"]
+    for k in ("u", "pos", "force", "U"):
+        assert np.array_equal(a[k], b[k]), k
