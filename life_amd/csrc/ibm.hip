@@ -177,6 +177,39 @@ __global__ void k_compute_ds_batch(const CaseDev* __restrict__ cases, const int*
 // A_ij of ObjectsClass::computeEpsilon (Objects.cpp:271-300) for the system
 // of markers [first, first+n): sum over the supports s of marker i of
 // delta_i(s) * delta(x_j - s), times ds_j.
+// The sum over the supports of marker mi of delta_s * d(xj - i_s) d(yj - j_s)
+// in support order (an epsilon matrix entry before the spacing factor,
+// Objects.cpp:271-300).  find_support keeps the sites with |x - i| < 1.5 and
+// |y - j| < 1.5 inside the lattice, scanned i-major in ascending order: a
+// block of at most three consecutive columns by three consecutive rows, the
+// first support being the lowest of each.  So the kernel values of the x
+// and y distances are evaluated once per column and per row (six
+// evaluations instead of eighteen) and picked per support: the same values,
+// products and sum as evaluating them support by support.
+__device__ __forceinline__ double support_sum(const MarkerDev& m, int mi, double xj, double yj) {
+    const int cnt = m.supp_cnt[mi];
+    int sij[2 * kSupp];
+    double sd[kSupp];
+#pragma unroll
+    for (int s = 0; s < 2 * kSupp; s++) sij[s] = m.supp_ij[mi * kSupp * 2 + s];
+#pragma unroll
+    for (int s = 0; s < kSupp; s++) sd[s] = m.supp_delta[mi * kSupp + s];
+    const int i0 = sij[0], j0 = sij[1];
+    const double x0 = dirac64(fabs(xj - i0)), x1 = dirac64(fabs(xj - (i0 + 1))), x2 = dirac64(fabs(xj - (i0 + 2)));
+    const double y0 = dirac64(fabs(yj - j0)), y1 = dirac64(fabs(yj - (j0 + 1))), y2 = dirac64(fabs(yj - (j0 + 2)));
+    double a = 0.0;
+#pragma unroll
+    for (int s = 0; s < kSupp; s++) {
+        if (s < cnt) {
+            const int kx = sij[2 * s] - i0, ky = sij[2 * s + 1] - j0;
+            const double dx = kx == 0 ? x0 : (kx == 1 ? x1 : x2);
+            const double dy = ky == 0 ? y0 : (ky == 1 ? y1 : y2);
+            a += sd[s] * (dx * dy);
+        }
+    }
+    return a;
+}
+
 __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, double* __restrict__ A) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * n) return;
@@ -184,14 +217,7 @@ __global__ void k_epsilon_matrix(MarkerDev m, GridScalars g, int first, int n, d
     const int mi = first + r, mj = first + c;
     const double Dx = g.Dx;
     const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx;
-    double a = 0.0;
-    const int cnt = m.supp_cnt[mi];
-    for (int s = 0; s < cnt; s++) {
-        const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
-        const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
-        a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
-    }
-    A[t] = a * m.ds[mj];
+    A[t] = support_sum(m, mi, xj, yj) * m.ds[mj];
 }
 
 // ---------------------------------------------------------------- epsilon, banded
@@ -216,22 +242,7 @@ __device__ inline void eps_band_entry(const MarkerDev& m, const GridScalars& g, 
         // every read issued before the arithmetic: the support slots past
         // the count hold zeros and are read but not used
         const double xj = m.pos[2 * mj] / Dx, yj = m.pos[2 * mj + 1] / Dx, dsj = m.ds[mj];
-        const int cnt = m.supp_cnt[mi];
-        int sij[2 * kSupp];
-        double sd[kSupp];
-#pragma unroll
-        for (int s = 0; s < 2 * kSupp; s++) sij[s] = m.supp_ij[mi * kSupp * 2 + s];
-#pragma unroll
-        for (int s = 0; s < kSupp; s++) sd[s] = m.supp_delta[mi * kSupp + s];
-#pragma unroll
-        for (int s = 0; s < kSupp; s++) {
-            if (s < cnt) {
-                const double dx = fabs(xj - sij[2 * s]);
-                const double dy = fabs(yj - sij[2 * s + 1]);
-                a += sd[s] * (dirac64(dx) * dirac64(dy));
-            }
-        }
-        a = a * dsj;
+        a = support_sum(m, mi, xj, yj) * dsj;
     }
     band[t] = a;
 }
@@ -976,13 +987,7 @@ __device__ __forceinline__ void recompute_body(const MarkerDev& m, const GridSca
         if (q <= 2 * KB && c >= 0 && c < n) {
             const int mi = first + r;
             const double xj = P[2 * c] / Dx, yj = P[2 * c + 1] / Dx;
-            const int cnt = m.supp_cnt[mi];
-            for (int s = 0; s < cnt; s++) {
-                const double dx = fabs(xj - m.supp_ij[(mi * kSupp + s) * 2]);
-                const double dy = fabs(yj - m.supp_ij[(mi * kSupp + s) * 2 + 1]);
-                a += m.supp_delta[mi * kSupp + s] * (dirac64(dx) * dirac64(dy));
-            }
-            a = a * dsl[c];
+            a = support_sum(m, mi, xj, yj) * dsl[c];
         }
         Bl[t] = a;
         if (a != 0.0) bw = max(bw, q < KB ? KB - q : q - KB);
