@@ -60,3 +60,53 @@ def schafer_turek_run(test, res_factor, seconds=20.0, precision=L.LIFE_FP64, sam
         c.close()
     out["wall_s"] = time.perf_counter() - t0
     return out
+
+
+# Published FSI2 results (Turek & Hron 2006, "Proposal for numerical
+# benchmarking of fluid-structure interaction between an elastic object and
+# laminar incompressible flow"): displacement of point A (the trailing edge
+# of the flag) in mm as mean +- amplitude, and the frequency of u_y in Hz.
+TUREK_HRON_FSI2 = {"ux_mean": -14.58, "ux_amp": 12.44, "uy_mean": 1.23, "uy_amp": 80.6, "f_y": 2.0}
+
+
+def turek_hron_run(res_factor, seconds=12.0, window_s=3.0, precision=L.LIFE_FP64, sample_s=0.005):
+    """The reference's TurekHron example (examples/TurekHron/params.h) at
+    resolution factor r (Nx = 250 r + 1, Ny = 41 r + 1, tStep = 0.001 / r^2):
+    displacement of the flag's last marker (point A, at (0.6, 0.2)) sampled
+    every `sample_s` seconds; mean and amplitude of u_x and u_y in mm over
+    the last `window_s` seconds and the frequencies of u_y and u_x from their
+    upward crossings of the window mean."""
+    import time
+    p, g = cfg.reference_example("TurekHron")
+    r = int(res_factor)
+    p = p.replace(Nx=r * 250 + 1, Ny=r * 41 + 1, tStep=0.001 / (r * r))
+    c = Case(p, g, precision=precision)
+    steps = int(round(seconds / p.tStep))
+    every = int(round(sample_s / p.tStep))
+    t0 = time.perf_counter()
+    try:
+        b = c.body_info(1)
+        tip = b["first"] + b["n"] - 1
+        a0 = c.markers()["pos"][tip].copy()
+        ts, ux, uy = [], [], []
+        for k in range(steps // every):
+            c.step(every)
+            q = c.markers()["pos"][tip]
+            ts.append((k + 1) * every * p.tStep)
+            ux.append(q[0] - a0[0])
+            uy.append(q[1] - a0[1])
+    finally:
+        c.close()
+    ts, ux, uy = np.array(ts), 1e3 * np.array(ux), 1e3 * np.array(uy)
+    w = ts > ts[-1] - window_s
+
+    def freq(v):
+        z = v[w] - v[w].mean()
+        tw = ts[w]
+        up = [tw[k] - z[k] * (tw[k + 1] - tw[k]) / (z[k + 1] - z[k]) for k in range(len(z) - 1) if z[k] < 0.0 <= z[k + 1]]
+        return (len(up) - 1) / (up[-1] - up[0]) if len(up) > 2 else None
+
+    return {"res_factor": r, "steps": steps, "point_A": a0.tolist(),
+            "ux_mean": float((ux[w].max() + ux[w].min()) / 2), "ux_amp": float((ux[w].max() - ux[w].min()) / 2),
+            "uy_mean": float((uy[w].max() + uy[w].min()) / 2), "uy_amp": float((uy[w].max() - uy[w].min()) / 2),
+            "f_y": freq(uy), "f_x": freq(ux), "wall_s": time.perf_counter() - t0}
