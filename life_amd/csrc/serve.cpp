@@ -188,9 +188,12 @@ struct life_server {
     hipGraphExec_t exec_bulk = nullptr;
     // early bulk stream-collide of the next step (environment
     // LIFE_SERVE_EARLY): 0 off, 1 launched when the lattice phase is
-    // queued, 2 launched when the host FEM of the first subiteration starts
-    // (the device is otherwise idle then)
-    int early = 2;
+    // queued (beside the whole coupling), 2 launched when the FEM of the
+    // first subiteration starts (which paid while the FEM ran on host
+    // threads and left the device idle; with the device FEM, capacity run of
+    // 256 cases x 2000 steps: 207 000-208 100 cases/hour with 1, 181 500-
+    // 182 800 with 2)
+    int early = 1;
     bool prepushed = false;         // this step's bulk stream-collide already ran on stream2
     hipStream_t stream2 = nullptr;  // shared per device (masked_stream), not owned
     hipEvent_t ev_lattice = nullptr, ev_bulk = nullptr;
@@ -917,10 +920,11 @@ int life_server_create(const life_server_config* cfg, life_server** out) {
         LIFE_HIP(hipSetDevice(cfg->device));
         LIFE_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         // CUs the early bulk kernel leaves to the coupling kernels beside it
-        // (environment LIFE_SERVE_COUPLING_CUS, default 16 of 256: measured
-        // best on MI355X among 0, 16 and 32)
+        // (environment LIFE_SERVE_COUPLING_CUS, default 32 of 256; capacity
+        // run with the early launch at the lattice phase: 207 500-208 100
+        // cases/hour with 32, 206 600-207 200 with 16, 190 700-191 800 with 0)
         const char* rc_env = std::getenv("LIFE_SERVE_COUPLING_CUS");
-        s->stream2 = masked_stream(cfg->device, rc_env ? std::atoi(rc_env) : 16);
+        s->stream2 = masked_stream(cfg->device, rc_env ? std::atoi(rc_env) : 32);
         LIFE_HIP(hipEventCreateWithFlags(&s->ev_lattice, hipEventDisableTiming));
         LIFE_HIP(hipEventCreateWithFlags(&s->ev_bulk, hipEventDisableTiming));
         if (const char* e = std::getenv("LIFE_SERVE_EARLY")) s->early = std::atoi(e);

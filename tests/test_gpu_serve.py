@@ -127,7 +127,7 @@ def test_womersley_served_matches_case(prec):
 def test_early_bulk_matches_plain(monkeypatch, prec, graph):
     """The next step's bulk stream-collide launched beside the coupling
     (while the set of cases stays fixed), as soon as the lattice phase is
-    queued (mode 1) or when the host FEM starts (mode 2, the default), gives
+    queued (mode 1, the default) or when the FEM starts (mode 2), gives
     the same results, bit for bit, as the plain order, through admissions
     and retirements."""
     pw, gw = cfg.womersley_cylinder()
