@@ -197,6 +197,10 @@ struct life_server {
     bool prepushed = false;         // this step's bulk stream-collide already ran on stream2
     hipStream_t stream2 = nullptr;  // shared per device (masked_stream), not owned
     hipEvent_t ev_lattice = nullptr, ev_bulk = nullptr;
+    // the last upload of the bulk case table (cases_bulk) from its
+    // page-locked host copy; the host rewrites that copy only after it
+    hipEvent_t ev_bulk_up = nullptr;
+    bool bulk_up_pending = false;
     Mirror<CaseDev> cases_bulk;     // descriptors the bulk stream-collide reads
     double early_steps = 0.0;
     // FEM on the device (FP32-shifted populations): every case's flexible
@@ -455,6 +459,13 @@ void run_phase(life_server* s, hipGraph_t& graph, hipGraphExec_t& exec, hipStrea
     LIFE_HIP(hipGraphLaunch(exec, on));
 }
 
+// Wait until the last upload of the bulk case table has read its host copy.
+void bulk_table_free(life_server* s) {
+    if (!s->bulk_up_pending) return;
+    LIFE_HIP(hipEventSynchronize(s->ev_bulk_up));
+    s->bulk_up_pending = false;
+}
+
 void lattice_phase(life_server* s) {
     if (s->prepushed) {
         // the bulk stream-collide of this step ran on stream2 beside the
@@ -462,8 +473,11 @@ void lattice_phase(life_server* s) {
         LIFE_HIP(hipStreamWaitEvent(s->stream, s->ev_bulk, 0));
         s->prepushed = false;
     } else {
+        bulk_table_free(s);
         std::copy(s->cases.h, s->cases.h + s->slots.size(), s->cases_bulk.h);
         s->cases_bulk.upload(s->slots.size(), s->stream);
+        LIFE_HIP(hipEventRecord(s->ev_bulk_up, s->stream));
+        s->bulk_up_pending = true;
         run_phase(s, s->graph_bulk, s->exec_bulk, s->stream, [&](hipStream_t st) { launch_bulk(s, st); });
     }
     run_phase(s, s->graph, s->exec, s->stream, [&](hipStream_t st) { launch_tail(s, st); });
@@ -479,6 +493,7 @@ void launch_bulk_early(life_server* s) {
     if (s->prepushed || s->slots.empty() || can_admit(s)) return;
     for (const Slot& sl : s->slots)
         if (sl.c->t >= sl.req->nsteps || sl.req->info.status == LIFE_REQ_FAILED) return;
+    bulk_table_free(s);
     for (int q = 0; q < (int)s->slots.size(); q++) {
         const life_case* c = s->slots[q].c;
         const life_grid* g = c->g;
@@ -489,9 +504,17 @@ void launch_bulk_early(life_server* s) {
         d.dst = g->f[1 - g->cur];
         d.explicit_state = 0;
     }
+    // The table goes up on stream2 behind the previous bulk kernel (the
+    // last reader of the old table) and ahead of the wait for this step's
+    // lattice phase: it is in place by the time that phase ends, and the
+    // bulk kernel starts on the event alone (a copy after the wait cost
+    // about 40 us per engine step between the boundaries and the bulk
+    // kernel).
+    s->cases_bulk.upload(s->slots.size(), s->stream2);
+    LIFE_HIP(hipEventRecord(s->ev_bulk_up, s->stream2));
+    s->bulk_up_pending = true;
     LIFE_HIP(hipEventRecord(s->ev_lattice, s->stream));
     LIFE_HIP(hipStreamWaitEvent(s->stream2, s->ev_lattice, 0));
-    s->cases_bulk.upload(s->slots.size(), s->stream2);
     run_phase(s, s->graph_bulk, s->exec_bulk, s->stream2, [&](hipStream_t st) { launch_bulk(s, st); });
     LIFE_HIP(hipEventRecord(s->ev_bulk, s->stream2));
     s->prepushed = true;
@@ -855,9 +878,17 @@ void engine_step(life_server* s) {
             fail(s, sl, e.what());  // retired below, its slot and memory freed
         }
     }
-    // a case that retires below may still be read by the early kernel, and
-    // cases_bulk's host copy is written again next step
-    LIFE_HIP(hipStreamSynchronize(s->stream2));
+    // A case that retires below may still be read by the early bulk kernel
+    // (launched only while no case finishes this step, so after a failure):
+    // the second stream is drained first.  Otherwise the host goes on to
+    // the next step while that kernel runs, and the next step's lattice
+    // phase waits for it on the device (draining it every step held the
+    // host's launches of the next tail until the bulk kernel had ended).
+    // The bulk table's host copy has its own event (bulk_table_free).
+    bool retiring = false;
+    for (const Slot& sl : s->slots)
+        retiring = retiring || sl.c->t >= sl.req->nsteps || sl.req->info.status == LIFE_REQ_FAILED;
+    if (retiring && s->prepushed) LIFE_HIP(hipStreamSynchronize(s->stream2));
     s->times.lap(s->stream, 5);
     s->times.steps += 1;
     s->steps += 1.0;
@@ -927,6 +958,7 @@ int life_server_create(const life_server_config* cfg, life_server** out) {
         s->stream2 = masked_stream(cfg->device, rc_env ? std::atoi(rc_env) : 32);
         LIFE_HIP(hipEventCreateWithFlags(&s->ev_lattice, hipEventDisableTiming));
         LIFE_HIP(hipEventCreateWithFlags(&s->ev_bulk, hipEventDisableTiming));
+        LIFE_HIP(hipEventCreateWithFlags(&s->ev_bulk_up, hipEventDisableTiming));
         if (const char* e = std::getenv("LIFE_SERVE_EARLY")) s->early = std::atoi(e);
         s->t0 = Clock::now();
     });
@@ -964,6 +996,7 @@ int life_server_destroy(life_server* s) {
         s->fem_out.release();
         if (s->ev_lattice) (void)hipEventDestroy(s->ev_lattice);
         if (s->ev_bulk) (void)hipEventDestroy(s->ev_bulk);
+        if (s->ev_bulk_up) (void)hipEventDestroy(s->ev_bulk_up);
         if (s->stream) (void)hipStreamDestroy(s->stream);
     });
     delete s;
