@@ -117,7 +117,7 @@ struct life_grid {
     hipStream_t tail_stream = nullptr;  // where the pending step tail runs
     hipEvent_t ev_ready = nullptr, ev_bulk = nullptr, ev_tail = nullptr;
     bool bulk_pending = false;  // a bulk kernel of the next step is queued on stream2
-    bool tail_pending = false;  // the last step's tail ran on stream2 (stream not yet joined)
+    bool tail_pending = false;  // the last step's tail ran on tail_stream (stream not yet joined)
     int t_step = 0;             // time step of the last lbm_step
     // blow-up scan (GridClass::writeInfo's NaN test): every check_every
     // steps (0: never) the populations are scanned on the device; the

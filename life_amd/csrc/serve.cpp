@@ -507,9 +507,9 @@ void launch_bulk_early(life_server* s) {
     // The table goes up on stream2 behind the previous bulk kernel (the
     // last reader of the old table) and ahead of the wait for this step's
     // lattice phase: it is in place by the time that phase ends, and the
-    // bulk kernel starts on the event alone (a copy after the wait cost
-    // about 40 us per engine step between the boundaries and the bulk
-    // kernel).
+    // bulk kernel starts on the event alone.  (A profiled run showed about
+    // 40 us between the boundaries and the bulk kernel with the copy after
+    // the wait; unprofiled capacity is the same within the spread.)
     s->cases_bulk.upload(s->slots.size(), s->stream2);
     LIFE_HIP(hipEventRecord(s->ev_bulk_up, s->stream2));
     s->bulk_up_pending = true;
