@@ -640,6 +640,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "kernel_ms": k_avg_s * 1e3, "kernel_mlups": N / k_avg_s / 1e6,
+                     "live": "beside the coupling chain; with the FEM on the device the bulk kernel runs on "
+                             "XCDs 2-7 (192 of 256 CUs) and the chain on XCDs 0-1 (LIFE_COUPLING_XCDS)",
                      "alone": {"steps": alone_steps, "kernel_ms": k_alone_s * 1e3,
                                "achieved": algo_bytes / k_alone_s / 1e9,
                                "frac": algo_bytes / k_alone_s / 1e9 / HBM_PEAK_GBS}},
