@@ -1188,6 +1188,33 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
     spread_reduce_one(m, sc, fibm, S, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The whole spread of a case with few marker-support pairs in one
+// workgroup: the clear, reset, count, scatter and reduce passes of the
+// separate launches in turn, each the same per-pair or per-site code,
+// separated by workgroup barriers (which make one pass's writes to the
+// scratch in device memory visible to the next).  Same pairs, same order at
+// each site, same sums; one launch instead of five, which is most of the
+// spread's time for a case of a few hundred markers.
+constexpr int kSpreadOneBlock = 1024;
+constexpr int kSpreadOneMaxPairs = 8192;
+
+__global__ __launch_bounds__(kSpreadOneBlock) void k_spread_one(MarkerDev m, SpreadScratch sc,
+                                                                double* __restrict__ fibm, long long S, int clear) {
+    coupling_priority();
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int np = m.n * kSupp;
+    if (clear)
+        for (int k = t; k < sc.cap; k += nt) spread_clear_one(fibm, S, sc.uniq, sc.uniq_n, k);
+    __syncthreads();
+    if (t == 0) *sc.uniq_n = 0;
+    __syncthreads();
+    for (int p = t; p < np; p += nt) spread_count_one(m, sc, p);
+    __syncthreads();
+    for (int p = t; p < np; p += nt) spread_scatter_one(m, sc, p);
+    __syncthreads();
+    for (int u = t; u < sc.cap; u += nt) spread_reduce_one(m, sc, fibm, S, u);
+}
+
 // Batched spread, grid.y = case (the reset: one thread per case).
 __global__ void k_spread_clear_batch(const CaseDev* __restrict__ cases, const int* __restrict__ list) {
     const CaseDev& c = cases[list[blockIdx.y]];
@@ -1316,6 +1343,11 @@ void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, Sprea
                    hipStream_t s) {
     const int np = m.n * kSupp;
     const int cap = sc.cap;
+    if (np > 0 && np <= kSpreadOneMaxPairs && cap >= np) {
+        if (dense_clear) launch_fill_f64(fibm, 0.0, g.S * 2, s);
+        k_spread_one<<<1, kSpreadOneBlock, 0, s>>>(m, sc, fibm, g.S, dense_clear ? 0 : 1);
+        return;
+    }
     if (dense_clear) {
         launch_fill_f64(fibm, 0.0, g.S * 2, s);
     } else if (cap > 0) {
