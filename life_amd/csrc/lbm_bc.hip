@@ -89,13 +89,27 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
     // entered from outside the domain) are never read, so only the known
     // ones are loaded: six of nine on an edge
     const bool corner = b.nx != 0 && b.ny != 0;
-    double fs[kNV];
+    // Every load the site needs is issued before the arithmetic (one round
+    // trip to memory, not up to three): its known populations, the two
+    // inward neighbours' populations (the zero gradient of free-slip and
+    // pressure sites, the density extrapolation of corners) and, at a
+    // pressure corner, its previous populations.
+    const bool nbr = b.type == 3 || b.type == 4 || corner;
+    const bool prev = b.type == 4 && corner && !explicit_state;
+    const long long id1 = (long long)(b.i + b.nx) * Ny + (j + b.ny);
+    const long long id2 = (long long)(b.i + 2 * b.nx) * Ny + (j + 2 * b.ny);
+    double fs[kNV], f1[kNV], f2[kNV], fp[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; v++) {
         const bool unknown = corner ? (kCX[v] == b.nx || kCY[v] == b.ny)
                                     : ((b.nd == 0 ? kCX[v] : kCY[v]) == (b.nd == 0 ? b.nx : b.ny));
         fs[v] = unknown ? 0.0 : Pops<T>::get(dst, S, v, id);
     }
+    if (nbr) {
+        load_site(dst, S, id1, f1);
+        load_site(dst, S, id2, f2);
+    }
+    if (prev) load_site(src, S, id, fp);
     // rho_n/u_n of the previous step outlive applyBCs + regularisedBC only
     // at a pressure corner: everywhere else all three are overwritten below
     // (wall/velocity set u, free-slip and pressure take u from the normal
@@ -109,14 +123,10 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
             uxn = u0[2 * id];
             uyn = u0[2 * id + 1];
         } else {
-            double fp[kNV];
-            load_site(src, S, id, fp);
             moments64(fp, (g.fprev ? g.fprev[id] : g.Fxm) + fibm[id], (g.fprev ? g.fprev[S + id] : g.Fym) + fibm[S + id],
                       rn, uxn, uyn);
         }
     }
-    const long long id1 = (long long)(b.i + b.nx) * Ny + (j + b.ny);
-    const long long id2 = (long long)(b.i + 2 * b.nx) * Ny + (j + 2 * b.ny);
     double rho1 = 0.0, rho2 = 0.0;
     switch (b.type) {
         case 1:  // wall
@@ -136,9 +146,7 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
                 rn = a.rho_in[j];
             }
             // Utils::zeroGradient order 2 on the current velocity (Utils.h:200-210)
-            double f1[kNV], f2[kNV], r1, r2, u1x, u1y, u2x, u2y;
-            load_site(dst, S, id1, f1);
-            load_site(dst, S, id2, f2);
+            double r1, r2, u1x, u1y, u2x, u2y;
             moments64(f1, g.fcur ? g.fcur[id1] : g.Fx, g.fcur ? g.fcur[S + id1] : g.Fy, r1, u1x, u1y);
             moments64(f2, g.fcur ? g.fcur[id2] : g.Fx, g.fcur ? g.fcur[S + id2] : g.Fy, r2, u2x, u2y);
             if (b.nd != 0) uxn = (4.0 / 3.0) * u1x - (1.0 / 3.0) * u2x;
@@ -148,10 +156,7 @@ __device__ inline void bc_site(const T* __restrict__ src, T* __restrict__ dst, c
         default:
             break;
     }
-    if (b.nx != 0 && b.ny != 0) {
-        double f1[kNV], f2[kNV];
-        load_site(dst, S, id1, f1);
-        load_site(dst, S, id2, f2);
+    if (corner) {
         rho1 = density64(f1);
         rho2 = density64(f2);
     }
