@@ -439,6 +439,13 @@ void object_kernel(life_case* c, EarlyBulk& eb, int mode) {
     c->subIt = 0;
     c->forces_on_host = false;
     PhaseTimes& T = *c->timing;
+    if (!c->hasFlex && mode != 2 && !T.on) {
+        // rigid bodies only: the spread follows the interpolation with
+        // nothing between them (one launch for a small case)
+        ibm_interp_spread(c->g);
+        T.lap(c->g, 4);
+        return;
+    }
     bool spread_final = false, more = false;
     do {
         if (c->hasFlex) recompute_object_vals(c);

@@ -1046,8 +1046,7 @@ constexpr int kInterpThreads = kInterpMarkers * kSupp;  // 252
 
 template <typename T>
 __device__ inline void interpolate_slots(const T* __restrict__ f, const MarkerDev& m, const GridScalars& g,
-                                         int k0, double* __restrict__ sh) {
-    const int t = threadIdx.x;
+                                         int k0, double* __restrict__ sh, int t) {
     const int km = t / kSupp, s = t - km * kSupp;
     const int k = k0 + km;
     double* a = sh;
@@ -1082,7 +1081,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_interpolate(const T* __restrict__ f, MarkerDev m, GridScalars g) {
     __shared__ double sh[3 * kInterpThreads];
     coupling_priority();
-    interpolate_slots(f, m, g, blockIdx.x * kInterpMarkers, sh);
+    interpolate_slots(f, m, g, blockIdx.x * kInterpMarkers, sh, (int)threadIdx.x);
 }
 
 // Batched: the populations after this step's stream-collide.
@@ -1091,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_interpolate_batch(const CaseDev* __rest
                                                            const int* __restrict__ list) {
     __shared__ double sh[3 * kInterpThreads];
     const CaseDev& c = cases[list[blockIdx.y]];
-    interpolate_slots(static_cast<const T*>(c.dst), c.m, c.g, blockIdx.x * kInterpMarkers, sh);
+    interpolate_slots(static_cast<const T*>(c.dst), c.m, c.g, blockIdx.x * kInterpMarkers, sh, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------- spread
@@ -1198,9 +1197,8 @@ __global__ void k_spread_reduce(MarkerDev m, SpreadScratch sc, double* __restric
 constexpr int kSpreadOneBlock = 1024;
 constexpr int kSpreadOneMaxPairs = 8192;
 
-__global__ __launch_bounds__(kSpreadOneBlock) void k_spread_one(MarkerDev m, SpreadScratch sc,
-                                                                double* __restrict__ fibm, long long S, int clear) {
-    coupling_priority();
+__device__ inline void spread_passes_wg(const MarkerDev& m, const SpreadScratch& sc, double* __restrict__ fibm,
+                                        long long S, int clear) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int np = m.n * kSupp;
     if (clear)
@@ -1213,6 +1211,31 @@ __global__ __launch_bounds__(kSpreadOneBlock) void k_spread_one(MarkerDev m, Spr
     for (int p = t; p < np; p += nt) spread_scatter_one(m, sc, p);
     __syncthreads();
     for (int u = t; u < sc.cap; u += nt) spread_reduce_one(m, sc, fibm, S, u);
+}
+
+__global__ __launch_bounds__(kSpreadOneBlock) void k_spread_one(MarkerDev m, SpreadScratch sc,
+                                                                double* __restrict__ fibm, long long S, int clear) {
+    coupling_priority();
+    spread_passes_wg(m, sc, fibm, S, clear);
+}
+
+// Rigid bodies of a small case, where the spread follows the interpolation
+// with nothing between them: both in one workgroup.  Four groups of 256
+// lanes take kInterpMarkers markers each per round with the interpolation's
+// own per-marker code (kInterpThreads of each group's lanes); a barrier;
+// then the spread passes as in k_spread_one.
+template <typename T>
+__global__ __launch_bounds__(kSpreadOneBlock) void k_interp_spread_one(const T* __restrict__ f, MarkerDev m,
+                                                                       GridScalars g, SpreadScratch sc,
+                                                                       double* __restrict__ fibm, int clear) {
+    __shared__ double sh[kSpreadOneBlock / 256][3 * kInterpThreads];
+    coupling_priority();
+    constexpr int kGroups = kSpreadOneBlock / 256;
+    const int grp = (int)threadIdx.x / 256, tl = (int)threadIdx.x % 256;
+    for (int k0 = 0; k0 < m.n; k0 += kGroups * kInterpMarkers)
+        interpolate_slots(f, m, g, k0 + grp * kInterpMarkers, sh[grp], tl);
+    __syncthreads();
+    spread_passes_wg(m, sc, fibm, g.S, clear);
 }
 
 // Batched spread, grid.y = case (the reset: one thread per case).
@@ -1358,6 +1381,20 @@ void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, Sprea
     k_spread_count<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
     k_spread_scatter<<<(np + 255) / 256, 256, 0, s>>>(m, sc);
     k_spread_reduce<<<(cap + 127) / 128, 128, 0, s>>>(m, sc, fibm, g.S);
+}
+
+bool launch_interp_spread_small(const void* f, bool f64, const MarkerDev& m, const GridScalars& g, double* fibm,
+                                SpreadScratch& sc, bool dense_clear, hipStream_t s) {
+    const int np = m.n * kSupp;
+    if (np <= 0 || np > kSpreadOneMaxPairs || sc.cap < np) return false;
+    if (dense_clear) launch_fill_f64(fibm, 0.0, g.S * 2, s);
+    if (f64)
+        k_interp_spread_one<double><<<1, kSpreadOneBlock, 0, s>>>(static_cast<const double*>(f), m, g, sc, fibm,
+                                                                  dense_clear ? 0 : 1);
+    else
+        k_interp_spread_one<float><<<1, kSpreadOneBlock, 0, s>>>(static_cast<const float*>(f), m, g, sc, fibm,
+                                                                 dense_clear ? 0 : 1);
+    return true;
 }
 
 // ---------------------------------------------------------------- batched launchers

@@ -102,6 +102,11 @@ template <typename T>
 void launch_interpolate(const T* f, const MarkerDev& m, const GridScalars& g, hipStream_t s);
 void launch_spread(const MarkerDev& m, const GridScalars& g, double* fibm, SpreadScratch& sc, bool dense_clear,
                    hipStream_t s);
+// Interpolation (populations f, FP64 or FP32-shifted) and then the spread in
+// one launch, for a case of at most 8192 marker-support pairs; false (and
+// nothing launched) otherwise.
+bool launch_interp_spread_small(const void* f, bool f64, const MarkerDev& m, const GridScalars& g, double* fibm,
+                                SpreadScratch& sc, bool dense_clear, hipStream_t s);
 
 // ---------------------------------------------------------------- serving
 // Batched forms of the kernels above for the serving engine.  Every

@@ -469,6 +469,19 @@ void ibm_spread(life_grid* g) {
     LIFE_HIP(hipGetLastError());
 }
 
+void ibm_interp_spread(life_grid* g) {
+    if (g->md.n <= 0) return;
+    lbm_join(g);
+    if (launch_interp_spread_small(g->f[g->cur], g->prec == kFP64, g->md, g->g, g->fibm, g->sc, g->dense_clear_next,
+                                   g->stream)) {
+        g->dense_clear_next = false;
+        LIFE_HIP(hipGetLastError());
+        return;
+    }
+    ibm_interpolate(g);
+    ibm_spread(g);
+}
+
 void ibm_find_support(life_grid* g, int flexible_only) {
     if (g->md.n <= 0) return;
     launch_support_ds(g->md, g->g, g->nbodies, flexible_only, g->stream);

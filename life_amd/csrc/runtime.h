@@ -189,6 +189,8 @@ void blowup_check(life_grid* g, bool wait);
 void blowup_cadence(life_grid* g, int t, hipStream_t s);
 void ibm_interpolate(life_grid* g);
 void ibm_spread(life_grid* g);
+// ibm_interpolate then ibm_spread, in one launch for a small case.
+void ibm_interp_spread(life_grid* g);
 void ibm_find_support(life_grid* g, int flexible_only);
 void ibm_epsilon_matrix(life_grid* g, int first, int n, double* A_host);
 // Declare the banded epsilon systems [first[k], first[k]+n[k]) once.
